@@ -1,0 +1,135 @@
+/*
+ * pf.h -- C-ABI of the MI355X phase-field timestep library (libpf.so).
+ *
+ * Drop-in boundary for the semi-implicit Fourier-spectral step of
+ * smjim/phase-field-microstructure-evolution (src/var_diff.f90).  A Fortran host
+ * binds these entry points with iso_c_binding (module pf_capi,
+ * phase-field-microstructure-evolution_amd/fortran/pf_capi.f90).
+ *
+ * Conventions
+ *   - Return value 0 on success, a negative PF_E* code otherwise; the message is
+ *     in pf_last_error() (thread-local).  The reference has no error returns (it
+ *     aborts inside P3DFFT/MPI); its Fortran host aborts on any non-zero code.
+ *   - Host arrays use the reference's Fortran layouts:
+ *       phi (var, nx, ny, nz), var fastest        src/var_diff.f90:154
+ *       con (nx, ny, nz), x fastest               src/var_diff.f90:143
+ *       half spectrum (nz, ny, nx/2+1), kz fastest (P3DFFT STRIDE1 Z-pencil,
+ *       src/var_diff.f90:135-137), complex as interleaved (re, im) doubles.
+ *   - Forward transforms use e^{-i} and divide by nx*ny*nz (src/f_trans.f90:11-12);
+ *     inverse transforms use e^{+i} and do not scale (src/inv_trans.f90:10).
+ *   - Each axis length must be one of 8 12 16 24 32 48 64 96 128 160 192 256 512.
+ *   - One host thread per handle; handles are independent.
+ */
+#ifndef PF_H
+#define PF_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_OK 0
+#define PF_EINVAL (-1)
+#define PF_ENOMEM (-2)
+#define PF_EHIP (-3)
+#define PF_ENOTIMPL (-4)
+#define PF_ENONFINITE (-5)
+
+#define PF_PREC_FP64 0
+#define PF_PREC_FP32 1
+
+#define PF_LAYOUT_XFAST 0 /* (nx/2+1, ny, nz), kx fastest (numpy rfftn order) */
+#define PF_LAYOUT_ZFAST 1 /* (nz, ny, nx/2+1), kz fastest (reference / P3DFFT order) */
+
+/* Step forms. */
+#define PF_MODE_FP64_FAITHFUL 0 /* the reference's loop sequence, FP64, real-space state */
+#define PF_MODE_FP64_REDUCED 1  /* same mathematics, spectral state, 2F transforms/step, FP64 */
+#define PF_MODE_FP32_MIXED 2    /* reduced form in FP32; FP64 k=0 modes and reductions */
+
+/* Parameters, named as in input.txt (src/var_diff.f90:106-113).  D_S is read by
+ * the reference but never used; D_mean = D_L (src/var_diff.f90:916).  gamma is
+ * built from sigma_1/sigma_2 exactly as src/var_diff.f90:206-218. */
+typedef struct pf_params {
+  double t_step, grad_coeff_phi, grad_coeff_c, D_S, D_L, D_gb, mob_phi;
+  double sigma_1, sigma_2, am, ap, con_0_mat, con_0_ppt;
+} pf_params;
+
+/* Diagnostics of the state at the START of step `step` (1-based, counted over the
+ * handle's lifetime): the energies the reference sums in loops 1-6 of that step
+ * and the extrema it computed at the end of the previous step -- i.e. the values
+ * of one summary.dat row (src/var_diff.f90:1058-1118).  c_mean is the mean of c
+ * (mass), reported in addition.  valid = 0 when the step was not diagnosed. */
+typedef struct pf_diag {
+  int step, valid;
+  double f_int, f_ch, f_eta, e_tot, c_max, c_min, phi_min, phi_3_max, c_mean;
+} pf_diag;
+
+typedef struct pf_handle pf_handle;
+
+/* ---- timestep ------------------------------------------------------------
+ * Replaces the body of the time loop (src/var_diff.f90:556-1118: loops 1-14,
+ * energies, extrema) and the P3DFFT setup/teardown (:128-138, :1162). */
+int pf_create(const pf_params* p, int nx, int ny, int nz, int var, int mode, pf_handle** out);
+int pf_destroy(pf_handle* h);
+/* Upload / download the state (src/var_diff.f90:154,143 layouts). */
+int pf_put_state(pf_handle* h, const double* phi, const double* con);
+int pf_get_state(pf_handle* h, double* phi, double* con);
+/* Advance nsteps steps.  out (nullable) receives nsteps records; a record is
+ * filled when its step number is a multiple of diag_every (diag_every <= 0: none).
+ * Returns PF_ENONFINITE if a diagnosed energy or mean is NaN/Inf. */
+int pf_step(pf_handle* h, int nsteps, int diag_every, pf_diag* out);
+/* Diagnostics of the current state (the start of the next step). */
+int pf_diagnose(pf_handle* h, pf_diag* out);
+/* Per-stage timing with HIP events on the handle's stream: enable/disable and
+ * read the accumulated milliseconds and launch counts of up to n stages. */
+int pf_set_timing(pf_handle* h, int on);
+int pf_stage_times(pf_handle* h, double* ms, long* count, int n);
+int pf_stage_name(int stage, char* buf, int len);
+/* Algorithmic HBM bytes of one launch of `stage` for this handle's grid. */
+double pf_stage_bytes(pf_handle* h, int stage);
+/* The HIP stream the handle launches on (hipStream_t). */
+void* pf_stream(pf_handle* h);
+
+/* ---- transforms ------------------------------------------------------------
+ * pf_fft_r2c: forward R2C of `batch` real fields (device buffers), output in
+ * `layout`, multiplied by `scale` (1/(nx*ny*nz) reproduces f_trans).
+ * pf_fft_c2r: inverse C2R, unscaled (inv_trans); imaginary parts of the kx=0
+ * and kx=nx/2 entries are ignored.  prec = PF_PREC_FP64 / PF_PREC_FP32.
+ * stream: hipStream_t or NULL. */
+int pf_fft_r2c(int nx, int ny, int nz, int batch, int prec, const void* d_in, void* d_out,
+               int layout, double scale, void* stream);
+int pf_fft_c2r(int nx, int ny, int nz, int batch, int prec, const void* d_in, void* d_out,
+               int layout, void* stream);
+/* Host-buffer FP64 drop-ins for f_trans(in,out,Nx,Ny,Nz,...) (src/f_trans.f90:1)
+ * and inv_trans(in,out,...) (src/inv_trans.f90:1); spectrum in PF_LAYOUT_ZFAST. */
+int pf_f_trans(const double* in, double* out, int nx, int ny, int nz);
+int pf_inv_trans(const double* in, double* out, int nx, int ny, int nz);
+
+/* ---- host helpers ----------------------------------------------------------- */
+/* ran_2 (src/ran_2.f90:1-40) with explicit, re-entrant state. */
+typedef struct pf_ran2_state {
+  int idum2, iy, iv[32];
+} pf_ran2_state;
+void pf_ran2_init(pf_ran2_state* st);
+double pf_ran2(pf_ran2_state* st, int* idum);
+/* k_space (src/k_space.f90:4-67): kf, kf_sq, kf_4 as (3, nz, ny, nx/2+1) complex. */
+int pf_k_space(int nx, int ny, int nz, double* kf, double* kf_sq, double* kf_4);
+/* gamma(var,var) column-major (src/var_diff.f90:198-218). */
+int pf_gamma(int var, double sigma_1, double sigma_2, double* gamma);
+/* Synthetic initial conditions (layouts as pf_put_state).  The draw orders are
+ * defined by this library (the reference has no such modes):
+ *   random : per point (z, y, x loops, x fastest) c first, then eta_1..eta_var;
+ *            c = c0 + c_amp*(2u-1), eta = eta_amp*(2u-1)
+ *   voronoi: nseeds seeds (x,y,z each u*N) first, nearest seed by periodic
+ *            minimum-image distance (lowest index on ties), eta_{(s mod var)+1}=1;
+ *            then c = c0 + c_amp*(2u-1) per point in the same order. */
+int pf_ic_random(int nx, int ny, int nz, int var, int iseed, double c0, double c_amp,
+                 double eta_amp, double* phi, double* con);
+int pf_ic_voronoi(int nx, int ny, int nz, int var, int nseeds, int iseed, double c0,
+                  double c_amp, double* phi, double* con);
+
+const char* pf_last_error(void);
+int pf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PF_H */

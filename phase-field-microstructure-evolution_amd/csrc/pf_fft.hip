@@ -1,0 +1,404 @@
+// 3-D real<->half-complex transforms on gfx950: twiddle tables, size dispatch,
+// whole-transform drivers, layout packing and the transform entry points of the
+// C-ABI (pf_fft_r2c / pf_fft_c2r on device buffers, pf_f_trans / pf_inv_trans
+// on host buffers with the reference's f_trans / inv_trans semantics).
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/pf.h"
+#include "pf_internal.hpp"
+#include "pf_kernels.hpp"
+
+namespace pf {
+
+#define PF_GRID_SIZES(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(160) X(192) X(256) X(512)
+
+bool grid_size_supported(int n) {
+  switch (n) {
+#define PF_CASE(N) case N:
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    return true;
+    default: return false;
+  }
+}
+const char* supported_sizes_text() { return "8 12 16 24 32 48 64 96 128 160 192 256 512"; }
+
+// ---------------------------------------------------------------------------
+// Twiddle tables.
+// ---------------------------------------------------------------------------
+void host_twiddles(int n, double* re, double* im) {
+  const long double two_pi = 6.283185307179586476925286766559L;
+  for (long t = 0; t < n; ++t) {
+    const long q = (4 * t) / n;           // quadrant
+    const long num = 4 * t - q * n;       // angle 2 pi num / (4n) in [0, pi/2)
+    long double c, s;
+    if (2 * num <= n) {
+      const long double a = two_pi * (long double)num / (long double)(4L * n);
+      c = cosl(a); s = sinl(a);
+    } else {
+      const long double a = two_pi * (long double)(n - num) / (long double)(4L * n);
+      c = sinl(a); s = cosl(a);
+    }
+    const double cr = (double)c, ci = -(double)s;  // e^{-i a}
+    double wr, wi;
+    switch (q) {  // times (-i)^q
+      case 0: wr = cr; wi = ci; break;
+      case 1: wr = ci; wi = -cr; break;
+      case 2: wr = -cr; wi = -ci; break;
+      default: wr = -ci; wi = cr; break;
+    }
+    re[t] = wr; im[t] = wi;
+  }
+}
+
+namespace {
+std::mutex g_tw_mu;
+std::map<std::pair<int, int>, void*> g_tw[2];  // [precision][(device, n)]
+}  // namespace
+
+template <typename R> const c2<R>* twiddle_table(int n) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  const int pi = sizeof(R) == 8 ? 1 : 0;
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto key = std::make_pair(dev, n);
+  auto it = g_tw[pi].find(key);
+  if (it != g_tw[pi].end()) return static_cast<const c2<R>*>(it->second);
+  std::vector<double> re(n), im(n);
+  host_twiddles(n, re.data(), im.data());
+  std::vector<c2<R>> h(n);
+  for (int t = 0; t < n; ++t) { h[t].x = (R)re[t]; h[t].y = (R)im[t]; }
+  void* d = nullptr;
+  if (hipMalloc(&d, sizeof(c2<R>) * n) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(c2<R>) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  g_tw[pi][key] = d;
+  return static_cast<const c2<R>*>(d);
+}
+template const c2<float>* twiddle_table<float>(int);
+template const c2<double>* twiddle_table<double>(int);
+
+// ---------------------------------------------------------------------------
+// Pass launchers with compile-time sizes.
+// ---------------------------------------------------------------------------
+static inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+template <typename R>
+int x_r2c(int nx, c2<R>* data, c2<R>* nyq, long nlines, R scale, hipStream_t s) {
+  const c2<R>* tw = twiddle_table<R>(nx);
+  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  switch (nx) {
+#define PF_CASE(N)                                                                      \
+  case N: {                                                                             \
+    constexpr int LPB = lines_per_block(N / 2);                                         \
+    hipLaunchKernelGGL((k_x_r2c<R, N>), dim3(ceil_div(nlines, LPB)),                    \
+                       dim3(LPB * fft_threads(N / 2)), 0, s, data, nyq, nlines, tw, scale); \
+    break;                                                                              \
+  }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported nx " + std::to_string(nx));
+  }
+  return 0;
+}
+
+template <typename R>
+int x_c2r(int nx, c2<R>* data, const c2<R>* nyq, long nlines, hipStream_t s) {
+  const c2<R>* tw = twiddle_table<R>(nx);
+  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  switch (nx) {
+#define PF_CASE(N)                                                                   \
+  case N: {                                                                          \
+    constexpr int LPB = lines_per_block(N / 2);                                      \
+    hipLaunchKernelGGL((k_x_c2r<R, N>), dim3(ceil_div(nlines, LPB)),                 \
+                       dim3(LPB * fft_threads(N / 2)), 0, s, data, nyq, nlines, tw); \
+    break;                                                                           \
+  }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported nx " + std::to_string(nx));
+  }
+  return 0;
+}
+
+template <typename R>
+int col_pass(int n, bool inv, c2<R>* data, long W, long npanels, hipStream_t s) {
+  const c2<R>* tw = twiddle_table<R>(n);
+  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  switch (n) {
+#define PF_CASE(N)                                                                          \
+  case N: {                                                                                 \
+    constexpr int TX = col_tile(N, sizeof(c2<R>));                                          \
+    const long tpp = (W + TX - 1) / TX;                                                     \
+    dim3 grid((unsigned)(tpp * npanels)), block(TX * fft_threads(N));                       \
+    if (inv)                                                                                \
+      hipLaunchKernelGGL((k_col_fft<R, N, true>), grid, block, 0, s, data, W, tpp, tw);    \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_col_fft<R, N, false>), grid, block, 0, s, data, W, tpp, tw);   \
+    break;                                                                                  \
+  }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported axis length " + std::to_string(n));
+  }
+  return 0;
+}
+
+template <typename R>
+int row_pass(int n, bool inv, c2<R>* data, long nlines, hipStream_t s) {
+  const c2<R>* tw = twiddle_table<R>(n);
+  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  switch (n) {
+#define PF_CASE(N)                                                                      \
+  case N: {                                                                             \
+    constexpr int LPB = lines_per_block(N);                                             \
+    dim3 grid(ceil_div(nlines, LPB)), block(LPB * fft_threads(N));                      \
+    if (inv)                                                                            \
+      hipLaunchKernelGGL((k_row_fft<R, N, true>), grid, block, 0, s, data, nlines, tw);  \
+    else                                                                                \
+      hipLaunchKernelGGL((k_row_fft<R, N, false>), grid, block, 0, s, data, nlines, tw); \
+    break;                                                                              \
+  }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported axis length " + std::to_string(n));
+  }
+  return 0;
+}
+
+#define PF_INST(R)                                                                \
+  template int x_r2c<R>(int, c2<R>*, c2<R>*, long, R, hipStream_t);               \
+  template int x_c2r<R>(int, c2<R>*, const c2<R>*, long, hipStream_t);            \
+  template int col_pass<R>(int, bool, c2<R>*, long, long, hipStream_t);           \
+  template int row_pass<R>(int, bool, c2<R>*, long, hipStream_t);
+PF_INST(float)
+PF_INST(double)
+#undef PF_INST
+
+// ---------------------------------------------------------------------------
+// Whole 3-D transforms (P3DFFT order: forward x, y, z; inverse z, y, x).
+// ---------------------------------------------------------------------------
+template <typename R>
+int fwd3d(c2<R>* data, c2<R>* nyq, int nx, int ny, int nz, int batch, R scale, hipStream_t s) {
+  const long M = nx / 2;
+  int rc;
+  if ((rc = x_r2c<R>(nx, data, nyq, (long)ny * nz * batch, scale, s))) return rc;
+  if ((rc = col_pass<R>(ny, false, data, M, (long)nz * batch, s))) return rc;
+  if ((rc = row_pass<R>(ny, false, nyq, (long)nz * batch, s))) return rc;
+  if ((rc = col_pass<R>(nz, false, data, M * ny, batch, s))) return rc;
+  if ((rc = col_pass<R>(nz, false, nyq, ny, batch, s))) return rc;
+  return 0;
+}
+
+template <typename R>
+int inv3d(c2<R>* data, c2<R>* nyq, int nx, int ny, int nz, int batch, hipStream_t s) {
+  const long M = nx / 2;
+  int rc;
+  if ((rc = col_pass<R>(nz, true, data, M * ny, batch, s))) return rc;
+  if ((rc = col_pass<R>(nz, true, nyq, ny, batch, s))) return rc;
+  if ((rc = col_pass<R>(ny, true, data, M, (long)nz * batch, s))) return rc;
+  if ((rc = row_pass<R>(ny, true, nyq, (long)nz * batch, s))) return rc;
+  if ((rc = x_c2r<R>(nx, data, nyq, (long)ny * nz * batch, s))) return rc;
+  return 0;
+}
+template int fwd3d<float>(c2<float>*, c2<float>*, int, int, int, int, float, hipStream_t);
+template int fwd3d<double>(c2<double>*, c2<double>*, int, int, int, int, double, hipStream_t);
+template int inv3d<float>(c2<float>*, c2<float>*, int, int, int, int, hipStream_t);
+template int inv3d<double>(c2<double>*, c2<double>*, int, int, int, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Split <-> dense half-spectrum layouts.
+//   XFAST: kx + (M+1)*(ky + ny*(kz + nz*f))     (numpy rfftn of a (nz,ny,nx) array)
+//   ZFAST: kz + nz*(ky + ny*(kx + (M+1)*f))     (P3DFFT STRIDE1 Z-pencil, as f_trans)
+// ---------------------------------------------------------------------------
+template <typename R, bool PACK>
+__global__ void k_layout(const c2<R>* __restrict__ a, c2<R>* __restrict__ b, c2<R>* __restrict__ nq,
+                         int M, int ny, int nz, long total, int layout) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long mx = M + 1;
+  long kx, ky, kz, f;
+  if (layout == LAYOUT_XFAST) {
+    kx = i % mx; long r = i / mx; ky = r % ny; r /= ny; kz = r % nz; f = r / nz;
+  } else {
+    kz = i % nz; long r = i / nz; ky = r % ny; r /= ny; kx = r % mx; f = r / mx;
+  }
+  const long plane = (long)ny * nz;
+  if (PACK) {
+    // a = main (split), nq = nyquist, b = dense
+    const c2<R> v = (kx < M) ? a[kx + M * (ky + ny * (kz + nz * f))] : nq[ky + ny * kz + plane * f];
+    b[i] = v;
+  } else {
+    // a = dense, b = main, nq = nyquist
+    const c2<R> v = a[i];
+    if (kx < M) b[kx + M * (ky + ny * (kz + nz * f))] = v;
+    else nq[ky + ny * kz + plane * f] = v;
+  }
+}
+
+template <typename R>
+int pack_spec(const c2<R>* main, const c2<R>* nyq, c2<R>* dense, int nx, int ny, int nz,
+              int batch, int layout, hipStream_t s) {
+  const long total = (long)(nx / 2 + 1) * ny * nz * batch;
+  hipLaunchKernelGGL((k_layout<R, true>), dim3(ceil_div(total, 256)), dim3(256), 0, s, main, dense,
+                     const_cast<c2<R>*>(nyq), nx / 2, ny, nz, total, layout);
+  return 0;
+}
+template <typename R>
+int unpack_spec(const c2<R>* dense, c2<R>* main, c2<R>* nyq, int nx, int ny, int nz, int batch,
+                int layout, hipStream_t s) {
+  const long total = (long)(nx / 2 + 1) * ny * nz * batch;
+  hipLaunchKernelGGL((k_layout<R, false>), dim3(ceil_div(total, 256)), dim3(256), 0, s, dense, main,
+                     nyq, nx / 2, ny, nz, total, layout);
+  return 0;
+}
+template int pack_spec<float>(const c2<float>*, const c2<float>*, c2<float>*, int, int, int, int, int, hipStream_t);
+template int pack_spec<double>(const c2<double>*, const c2<double>*, c2<double>*, int, int, int, int, int, hipStream_t);
+template int unpack_spec<float>(const c2<float>*, c2<float>*, c2<float>*, int, int, int, int, int, hipStream_t);
+template int unpack_spec<double>(const c2<double>*, c2<double>*, c2<double>*, int, int, int, int, int, hipStream_t);
+
+}  // namespace pf
+
+// ===========================================================================
+// C-ABI transform entry points.
+// ===========================================================================
+namespace {
+using namespace pf;
+
+int check_dims(int nx, int ny, int nz, int batch) {
+  if (!grid_size_supported(nx) || !grid_size_supported(ny) || !grid_size_supported(nz))
+    return fail(PF_EINVAL, std::string("grid ") + std::to_string(nx) + "x" + std::to_string(ny) +
+                               "x" + std::to_string(nz) + " not supported; each axis must be one of " +
+                               supported_sizes_text());
+  if (batch < 1) return fail(PF_EINVAL, "batch must be >= 1");
+  return 0;
+}
+
+struct Workspace {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mu;
+std::map<int, Workspace> g_ws;
+
+void* workspace(size_t bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Workspace& w = g_ws[dev];
+  if (w.bytes < bytes) {
+    if (w.p) (void)hipFree(w.p);
+    w.p = nullptr;
+    w.bytes = 0;
+    if (hipMalloc(&w.p, bytes) != hipSuccess) return nullptr;
+    w.bytes = bytes;
+  }
+  return w.p;
+}
+
+template <typename R>
+int r2c_dev(int nx, int ny, int nz, int batch, const void* in, void* out, int layout, double scale,
+            hipStream_t s) {
+  const size_t P = (size_t)nx * ny * nz, plane = (size_t)ny * nz;
+  char* ws = (char*)workspace(sizeof(R) * P * batch + sizeof(c2<R>) * plane * batch);
+  if (!ws) return fail(PF_ENOMEM, "workspace allocation failed");
+  c2<R>* main = (c2<R>*)ws;
+  c2<R>* nyq = (c2<R>*)(ws + sizeof(R) * P * batch);
+  PF_HIP(hipMemcpyAsync(main, in, sizeof(R) * P * batch, hipMemcpyDeviceToDevice, s));
+  int rc = fwd3d<R>(main, nyq, nx, ny, nz, batch, (R)scale, s);
+  if (rc) return rc;
+  rc = pack_spec<R>(main, nyq, (c2<R>*)out, nx, ny, nz, batch, layout, s);
+  if (rc) return rc;
+  PF_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int c2r_dev(int nx, int ny, int nz, int batch, const void* in, void* out, int layout, hipStream_t s) {
+  const size_t P = (size_t)nx * ny * nz, plane = (size_t)ny * nz;
+  char* ws = (char*)workspace(sizeof(R) * P * batch + sizeof(c2<R>) * plane * batch);
+  if (!ws) return fail(PF_ENOMEM, "workspace allocation failed");
+  c2<R>* main = (c2<R>*)ws;
+  c2<R>* nyq = (c2<R>*)(ws + sizeof(R) * P * batch);
+  int rc = unpack_spec<R>((const c2<R>*)in, main, nyq, nx, ny, nz, batch, layout, s);
+  if (rc) return rc;
+  rc = inv3d<R>(main, nyq, nx, ny, nz, batch, s);
+  if (rc) return rc;
+  PF_HIP(hipMemcpyAsync(out, main, sizeof(R) * P * batch, hipMemcpyDeviceToDevice, s));
+  PF_HIP(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
+extern "C" int pf_fft_r2c(int nx, int ny, int nz, int batch, int prec, const void* d_in, void* d_out,
+                          int layout, double scale, void* stream) {
+  int rc = check_dims(nx, ny, nz, batch);
+  if (rc) return rc;
+  if (!d_in || !d_out) return fail(PF_EINVAL, "null buffer");
+  if (layout != PF_LAYOUT_XFAST && layout != PF_LAYOUT_ZFAST) return fail(PF_EINVAL, "bad layout");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == PF_PREC_FP64) return r2c_dev<double>(nx, ny, nz, batch, d_in, d_out, layout, scale, s);
+  if (prec == PF_PREC_FP32) return r2c_dev<float>(nx, ny, nz, batch, d_in, d_out, layout, scale, s);
+  return fail(PF_EINVAL, "bad precision");
+}
+
+extern "C" int pf_fft_c2r(int nx, int ny, int nz, int batch, int prec, const void* d_in, void* d_out,
+                          int layout, void* stream) {
+  int rc = check_dims(nx, ny, nz, batch);
+  if (rc) return rc;
+  if (!d_in || !d_out) return fail(PF_EINVAL, "null buffer");
+  if (layout != PF_LAYOUT_XFAST && layout != PF_LAYOUT_ZFAST) return fail(PF_EINVAL, "bad layout");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == PF_PREC_FP64) return c2r_dev<double>(nx, ny, nz, batch, d_in, d_out, layout, s);
+  if (prec == PF_PREC_FP32) return c2r_dev<float>(nx, ny, nz, batch, d_in, d_out, layout, s);
+  return fail(PF_EINVAL, "bad precision");
+}
+
+// Host-buffer drop-ins for f_trans (src/f_trans.f90:1-15) and inv_trans
+// (src/inv_trans.f90:1-13): FP64, the spectrum in the reference's Z-pencil order.
+extern "C" int pf_f_trans(const double* in, double* out, int nx, int ny, int nz) {
+  int rc = check_dims(nx, ny, nz, 1);
+  if (rc) return rc;
+  if (!in || !out) return fail(PF_EINVAL, "null buffer");
+  const size_t P = (size_t)nx * ny * nz, H = (size_t)(nx / 2 + 1) * ny * nz;
+  void *din = nullptr, *dout = nullptr;
+  PF_HIP(hipMalloc(&din, sizeof(double) * P));
+  if (hipMalloc(&dout, sizeof(double) * 2 * H) != hipSuccess) {
+    (void)hipFree(din);
+    return fail(PF_ENOMEM, "device allocation failed");
+  }
+  rc = hipMemcpy(din, in, sizeof(double) * P, hipMemcpyHostToDevice) == hipSuccess ? 0 : -3;
+  const double scale = 1.0 / (double)(float)((long)nx * ny * nz);
+  if (!rc) rc = pf_fft_r2c(nx, ny, nz, 1, PF_PREC_FP64, din, dout, PF_LAYOUT_ZFAST, scale, nullptr);
+  if (!rc && hipMemcpy(out, dout, sizeof(double) * 2 * H, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(PF_EHIP, "copy back failed");
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return rc;
+}
+
+extern "C" int pf_inv_trans(const double* in, double* out, int nx, int ny, int nz) {
+  int rc = check_dims(nx, ny, nz, 1);
+  if (rc) return rc;
+  if (!in || !out) return fail(PF_EINVAL, "null buffer");
+  const size_t P = (size_t)nx * ny * nz, H = (size_t)(nx / 2 + 1) * ny * nz;
+  void *din = nullptr, *dout = nullptr;
+  PF_HIP(hipMalloc(&din, sizeof(double) * 2 * H));
+  if (hipMalloc(&dout, sizeof(double) * P) != hipSuccess) {
+    (void)hipFree(din);
+    return fail(PF_ENOMEM, "device allocation failed");
+  }
+  rc = hipMemcpy(din, in, sizeof(double) * 2 * H, hipMemcpyHostToDevice) == hipSuccess ? 0 : -3;
+  if (!rc) rc = pf_fft_c2r(nx, ny, nz, 1, PF_PREC_FP64, din, dout, PF_LAYOUT_ZFAST, nullptr);
+  if (!rc && hipMemcpy(out, dout, sizeof(double) * P, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(PF_EHIP, "copy back failed");
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return rc;
+}

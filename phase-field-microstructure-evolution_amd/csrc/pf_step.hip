@@ -1,0 +1,1004 @@
+// The timestep: handle, state I/O, the FP64 faithful sequence and the reduced
+// (spectral-state) FP64 / FP32-mixed sequences, diagnostics and stage timers.
+//
+// Field order inside the library: field 0 = c, fields 1..var = eta_1..eta_var.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pf.h"
+#include "pf_internal.hpp"
+#include "pf_step_kernels.hpp"
+
+namespace pf {
+double kaxis(int m, int n, bool zero_nyquist);
+
+enum Stage {
+  ST_NONLINEAR = 0, ST_X_R2C, ST_Y_FWD, ST_Z_FWD, ST_UPDATE, ST_Z_INV, ST_Y_INV, ST_X_C2R,
+  ST_DIAG, ST_MOBILITY, ST_GRAD, NSTAGE
+};
+static const char* kStageNames[NSTAGE] = {
+    "nonlinear", "x_r2c", "y_fwd", "z_fwd", "kspace_update", "z_inv", "y_inv", "x_c2r",
+    "diag_reduce", "mobility", "grad_energy"};
+
+constexpr int NB = 256;        // block size of pointwise kernels
+constexpr int NPART_NL = 6;    // f_ch, f_eta, c_max, c_min, phi_min, phi_3
+constexpr long kMaxGrid = 8192;
+
+struct UCoef {
+  double dt, kphi, kc, D, L, Dgb, am, ap, cm, cp, s1g, s2g, invP;
+};
+
+// ===========================================================================
+// Kernels
+// ===========================================================================
+
+// AoS host layout (phi var-fastest, con) <-> bank of real fields (c, eta_1..).
+template <typename R>
+__global__ void k_scatter_state(const double* __restrict__ phi, const double* __restrict__ con,
+                                R* __restrict__ dst, long P, int var) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < P; t += (long)gridDim.x * blockDim.x) {
+    dst[t] = (R)con[t];
+    for (int i = 0; i < var; ++i) dst[(long)(i + 1) * P + t] = (R)phi[t * var + i];
+  }
+}
+template <typename R>
+__global__ void k_gather_state(const R* __restrict__ src, double* __restrict__ phi,
+                               double* __restrict__ con, long P, int var) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < P; t += (long)gridDim.x * blockDim.x) {
+    con[t] = (double)src[t];
+    for (int i = 0; i < var; ++i) phi[t * var + i] = (double)src[(long)(i + 1) * P + t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Loops 5-10, faithful form (src/var_diff.f90:685-905), FP64, O(var^2).
+// Reads the real state bank (c, eta_i); writes df_dc, df_dphi_i, mob_c.
+// ---------------------------------------------------------------------------
+template <bool DIAG>
+__global__ void __launch_bounds__(NB)
+k_nl_faithful(Geo g, UCoef u, const double* __restrict__ gam, const double* __restrict__ st,
+              double* __restrict__ dfdc, double* __restrict__ dfdphi, double* __restrict__ mob,
+              double* __restrict__ part) {
+  extern __shared__ double sh[];  // var * NB values, then NB reduction scratch
+  double* red = sh + (long)g.var * NB;
+  const int V = g.var, mat = V - 1;
+  double a_ch = 0.0, a_eta = 0.0, cmax = 0.0, cmin = 1.0, pmin = 1.0, p3 = 0.0;
+  for (long base = (long)blockIdx.x * NB; base < g.P; base += (long)gridDim.x * NB) {
+    const long t = base + threadIdx.x;
+    const bool ok = t < g.P;
+    __syncthreads();
+    for (int i = 0; i < V; ++i) sh[i * NB + threadIdx.x] = ok ? st[(long)(i + 1) * g.P + t] : 0.0;
+    __syncthreads();
+    if (!ok) continue;
+    const double* ph = sh + threadIdx.x;
+    const double con = st[t];
+    // loop 5
+    double fe = 0.0;
+    for (int i = 0; i < V; ++i) {
+      const double fi = ph[i * NB], fi2 = fi * fi;
+      fe = (fe - 0.5 * fi2) + 0.25 * (fi2 * fi2);
+      for (int j = i + 1; j < V; ++j) {
+        const double fj = ph[j * NB];
+        fe = fe + ((gam[i + (long)V * j] * fi2) * (fj * fj));
+      }
+    }
+    // loop 6
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < V; ++i) {
+      const double f2 = ph[i * NB] * ph[i * NB], f4 = f2 * f2;
+      if (i < mat) num = num + f4;
+      den = den + f4;
+    }
+    const double hm = num / den, hp = 1.0 - hm;
+    const double dm = con - u.cm, dp = con - u.cp;
+    const double fch = (hm * u.am) * (dm * dm) + (hp * u.ap) * (dp * dp);
+    dfdc[t] = ((2.0 * hm) * u.am) * dm + ((2.0 * hp) * u.ap) * dp;
+    // loops 7-9
+    const double fm = u.am * (dm * dm), fp = u.ap * (dp * dp);
+    const double fv = ph[mat * NB], fv2 = fv * fv, fv4 = fv2 * fv2;
+    const double t2v = (4.0 * (fv2 * fv)) * (den - fv4);
+    const double den2 = den * den;
+    double ps = 0.0;
+    for (int i = 0; i < V; ++i) {
+      const double fi = ph[i * NB];
+      const double term1 = (fi * fi) * fi - fi;
+      double term2 = 0.0;
+      for (int j = 0; j < V; ++j) {
+        if (j == i) continue;
+        const double fj = ph[j * NB];
+        term2 = term2 + ((2.0 * gam[i + (long)V * j]) * fi) * (fj * fj);
+      }
+      const double term4 = (i < mat) ? (((4.0 * ((fi * fi) * fi)) * fv4) * (fm - fp)) / den2
+                                     : (t2v * (fp - fm)) / den2;
+      dfdphi[(long)i * g.P + t] = (term1 + term2) + term4;
+      ps = ps + fi * fi;
+    }
+    // loop 10: mob_c = D_bulk(var) + 2(1 - sum phi^2) D_gb, D_bulk(var) = D_L
+    mob[t] = u.D + ((2.0 * (1.0 - ps)) * u.Dgb);
+    if (DIAG) {
+      a_ch += fch;
+      a_eta += fe;
+      if (fv > p3) p3 = fv;
+      if (con <= cmin) cmin = con;
+      if (con > cmax) cmax = con;
+      if (ps < pmin) pmin = ps;
+    }
+  }
+  if (DIAG) {
+    __syncthreads();
+    const double s0 = block_sum<NB>(a_ch, red), s1 = block_sum<NB>(a_eta, red);
+    const double e0 = block_ext<NB, true>(cmax, red), e1 = block_ext<NB, false>(cmin, red);
+    const double e2 = block_ext<NB, false>(pmin, red), e3 = block_ext<NB, true>(p3, red);
+    if (threadIdx.x == 0) {
+      double* o = part + (long)blockIdx.x * NPART_NL;
+      o[0] = s0; o[1] = s1; o[2] = e0; o[3] = e1; o[4] = e2; o[5] = e3;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Loops 5-10, reduced form: O(var) per point using the block structure of gamma
+// (sigma_1 between matrix grains, sigma_2 between a grain and eta_var,
+// src/var_diff.f90:206-218).  In place on the real work bank:
+// (c, eta_1..eta_V) -> (df_dc, df_dphi_1..df_dphi_V); optional (m - D_mean).
+// ---------------------------------------------------------------------------
+template <typename R, bool DIAG, bool MOB, bool WRITE = true>
+__global__ void __launch_bounds__(NB)
+k_nl_reduced(Geo g, UCoef u, R* __restrict__ w, R* __restrict__ mobd, double* __restrict__ part) {
+  extern __shared__ unsigned char shraw[];
+  R* sh = reinterpret_cast<R*>(shraw);
+  double* red = reinterpret_cast<double*>(shraw + sizeof(R) * (size_t)g.var * NB + 16);
+  const int V = g.var, mat = V - 1;
+  const R am = (R)u.am, ap = (R)u.ap, cm = (R)u.cm, cp = (R)u.cp, s1g = (R)u.s1g, s2g = (R)u.s2g;
+  double a_ch = 0.0, a_eta = 0.0, cmax = 0.0, cmin = 1.0, pmin = 1.0, p3 = 0.0;
+  for (long base = (long)blockIdx.x * NB; base < g.P; base += (long)gridDim.x * NB) {
+    const long t = base + threadIdx.x;
+    const bool ok = t < g.P;
+    __syncthreads();
+    for (int i = 0; i < V; ++i) sh[i * NB + threadIdx.x] = ok ? w[(long)(i + 1) * g.P + t] : R(0);
+    __syncthreads();
+    if (!ok) continue;
+    const R* ph = sh + threadIdx.x;
+    const R con = w[t];
+    R sm = 0, num = 0, pair = 0;
+    R fe = 0;
+    for (int i = 0; i < mat; ++i) {
+      const R a = ph[i * NB] * ph[i * NB];
+      pair += a * sm;
+      sm += a;
+      num += a * a;
+      if (DIAG) fe += a * (R(0.25) * a - R(0.5));
+    }
+    const R fv = ph[mat * NB], av = fv * fv, av2 = av * av;
+    const R d4 = num + av2;
+    const R hm = num / d4, hp = R(1) - hm;
+    const R dm = con - cm, dp = con - cp;
+    const R fm = am * (dm * dm), fpp = ap * (dp * dp);
+    if (WRITE) w[t] = R(2) * hm * am * dm + R(2) * hp * ap * dp;
+    const R inv_d4 = R(1) / d4;
+    const R cT4m = R(4) * (fm - fpp) * (av2 * inv_d4) * inv_d4;
+    if (WRITE) {
+      for (int i = 0; i < mat; ++i) {
+        const R fi = ph[i * NB], a = fi * fi;
+        const R ex = s1g * (sm - a) + s2g * av;
+        w[(long)(i + 1) * g.P + t] = fi * (a - R(1)) + R(2) * fi * ex + cT4m * fi * a;
+      }
+      const R T4v = R(4) * fv * av * hm * (fpp - fm) * inv_d4;
+      w[(long)V * g.P + t] = fv * (av - R(1)) + R(2) * fv * s2g * sm + T4v;
+    }
+    const R ps = sm + av;
+    if (MOB && WRITE) mobd[t] = R(2) * (R(1) - ps) * (R)u.Dgb;
+    if (DIAG) {
+      fe += av * (R(0.25) * av - R(0.5)) + s1g * pair + s2g * av * sm;
+      a_ch += (double)(hm * fm + hp * fpp);
+      a_eta += (double)fe;
+      const double dfv = fv, dc = con, dps = ps;
+      if (dfv > p3) p3 = dfv;
+      if (dc <= cmin) cmin = dc;
+      if (dc > cmax) cmax = dc;
+      if (dps < pmin) pmin = dps;
+    }
+  }
+  if (DIAG) {
+    __syncthreads();
+    const double s0 = block_sum<NB>(a_ch, red), s1 = block_sum<NB>(a_eta, red);
+    const double e0 = block_ext<NB, true>(cmax, red), e1 = block_ext<NB, false>(cmin, red);
+    const double e2 = block_ext<NB, false>(pmin, red), e3 = block_ext<NB, true>(p3, red);
+    if (threadIdx.x == 0) {
+      double* o = part + (long)blockIdx.x * NPART_NL;
+      o[0] = s0; o[1] = s1; o[2] = e0; o[3] = e1; o[4] = e2; o[5] = e3;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Faithful k-space kernels (FP64).  One block per (ky, kz, field) row, threads
+// over kx in [0, M] (kx = M: Nyquist plane).  Grid-stride over rows.
+// ---------------------------------------------------------------------------
+// loops 1 and 3: dft_grad_d = dft * kf_d  (src/var_diff.f90:577-579, 645-647)
+__global__ void k_grad_spec(Geo g, KTab kt, Bank<double> st, Bank<double> gr, long nrows) {
+  for (long row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const RowIdx r = row_decode(g, row);
+    for (int kx = threadIdx.x; kx <= g.M; kx += blockDim.x) {
+      const double2 v = *elem(st, g, r.f, kx, r.ky, r.kz);
+      const double a[3] = {kt.axp[kx], kt.ayp[r.ky], kt.azp[r.kz]};
+      for (int d = 0; d < 3; ++d)
+        *elem(gr, g, 3 * r.f + d, kx, r.ky, r.kz) = make_double2(-(v.y * a[d]), v.x * a[d]);
+    }
+  }
+}
+
+// loops 2 and 4: f_int += 0.5*kappa*((gx^2+gy^2)+gz^2)  (src/var_diff.f90:620-622, 670-671)
+__global__ void __launch_bounds__(NB)
+k_fint_faithful(Geo g, UCoef u, const double* __restrict__ gr, double* __restrict__ part) {
+  __shared__ double red[NB];
+  double acc = 0.0;
+  const long total = (long)g.F * g.P;
+  for (long q = (long)blockIdx.x * NB + threadIdx.x; q < total; q += (long)gridDim.x * NB) {
+    const long f = q / g.P, t = q - f * g.P;
+    const double* b = gr + 3 * f * g.P + t;
+    const double gx = b[0], gy = b[g.P], gz = b[2 * g.P];
+    const double kap = (f == 0) ? u.kc : u.kphi;
+    acc += (0.5 * kap) * ((gx * gx + gy * gy) + gz * gz);
+  }
+  const double s = block_sum<NB>(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// loop 11: dft_dummy = kf_sum*dft_df_dc - grad_coeff_c*dft_con*k_cube  (src/var_diff.f90:932-936)
+template <typename R>
+__global__ void k_ghat(Geo g, UCoef u, KTab kt, Bank<R> fhat, int ff, Bank<R> chat, int cf,
+                       Bank<R> out, long nrows) {
+  for (long row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const RowIdx r = row_decode(g, row);
+    for (int kx = threadIdx.x; kx <= g.M; kx += blockDim.x) {
+      const KCoef k = kcoef(kt, kx, r.ky, r.kz);
+      const c2<R> f = *elem(fhat, g, ff, kx, r.ky, r.kz);
+      const c2<R> c = *elem(chat, g, cf, kx, r.ky, r.kz);
+      const double kcr = u.kc * (double)c.x, kci = u.kc * (double)c.y;
+      const double re = -(k.s1 * (double)f.y) - (-(kci * k.k3));
+      const double im = (k.s1 * (double)f.x) - (kcr * k.k3);
+      *elem(out, g, 0, kx, r.ky, r.kz) = mk<R>((R)re, (R)im);
+    }
+  }
+}
+
+// loop 12: dummy = dummy*(mob_c - D_mean)  (src/var_diff.f90:956)
+__global__ void k_mult_faithful(long P, double D, double* __restrict__ x, const double* __restrict__ mob) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < P; t += (long)gridDim.x * blockDim.x)
+    x[t] = x[t] * (mob[t] - D);
+}
+template <typename R>
+__global__ void k_mult(long P, R* __restrict__ x, const R* __restrict__ m) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < P; t += (long)gridDim.x * blockDim.x)
+    x[t] = x[t] * m[t];
+}
+
+// loop 13: dft_con = (dft_con + term11 + term33) / term22  (src/var_diff.f90:978-987)
+__global__ void k_ch_faithful(Geo g, UCoef u, KTab kt, Bank<double> st, Bank<double> hhat,
+                              Bank<double> fhat, long nrows) {
+  for (long row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const RowIdx r = row_decode(g, row);
+    for (int kx = threadIdx.x; kx <= g.M; kx += blockDim.x) {
+      const KCoef k = kcoef(kt, kx, r.ky, r.kz);
+      double2* cp = elem(st, g, 0, kx, r.ky, r.kz);
+      const double2 c = *cp, h = *elem(hhat, g, 0, kx, r.ky, r.kz), f = *elem(fhat, g, 0, kx, r.ky, r.kz);
+      const double t11r = -((h.y * u.dt) * k.s1), t11i = (h.x * u.dt) * k.s1;
+      const double t22 = 1.0 + ((u.D * u.dt) * u.kc) * k.k4;
+      const double r33 = (u.D * k.k2) * u.dt;
+      const double nr = (c.x + t11r) + r33 * f.x, ni = (c.y + t11i) + r33 * f.y;
+      *cp = make_double2(nr / t22, ni / t22);
+    }
+  }
+}
+
+// loop 14: dft_dummy = (dft_dummy - mob_phi*t_step*dft_dummy2) / (1 - mob_phi*kappa*t_step*k_sq)
+// (src/var_diff.f90:1019-1027); fields 1..var of the state bank, df_dphi bank 0..var-1.
+__global__ void k_ac_faithful(Geo g, UCoef u, KTab kt, Bank<double> st, Bank<double> dpsi, long nrows) {
+  const double ldt = u.L * u.dt, lkd = (u.L * u.kphi) * u.dt;
+  for (long row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const RowIdx r = row_decode(g, row);  // r.f in [0, var)
+    for (int kx = threadIdx.x; kx <= g.M; kx += blockDim.x) {
+      const KCoef k = kcoef(kt, kx, r.ky, r.kz);
+      double2* sp = elem(st, g, r.f + 1, kx, r.ky, r.kz);
+      const double2 s = *sp, p = *elem(dpsi, g, r.f, kx, r.ky, r.kz);
+      const double t22 = 1.0 - lkd * k.k2;
+      *sp = make_double2((s.x - ldt * p.x) / t22, (s.y - ldt * p.y) / t22);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reduced k-space update for all fields, in place on S, copy into W for the
+// inverse transform; optional Parseval partial sums of the new state for the
+// next step's f_int.  FP32 form keeps each field's k = 0 mode in FP64 (dc64).
+// ---------------------------------------------------------------------------
+template <typename R, bool DC64, bool MOB, bool PARSEVAL, bool UPD>
+__global__ void __launch_bounds__(NB)
+k_update(Geo g, UCoef u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> H, double* __restrict__ dc64,
+         double* __restrict__ part, long nrows) {
+  __shared__ double red[NB];
+  double acc = 0.0;
+  for (long row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const RowIdx r = row_decode(g, row);
+    const double kap = (r.f == 0) ? u.kc : u.kphi;
+    for (int kx = threadIdx.x; kx <= g.M; kx += blockDim.x) {
+      const KCoef k = kcoef(kt, kx, r.ky, r.kz);
+      c2<R>* sp = elem(S, g, r.f, kx, r.ky, r.kz);
+      c2<R> sn = *sp;
+      if (UPD) {
+        const c2<R> w = *elem(W, g, r.f, kx, r.ky, r.kz);
+        if (r.f == 0) {
+          const double den = 1.0 + ((u.D * u.dt) * u.kc) * k.k4;
+          const double A = 1.0 / den, B = ((u.D * k.k2) * u.dt) * A;
+          double re = A * (double)sn.x + B * (double)w.x;
+          double im = A * (double)sn.y + B * (double)w.y;
+          if (MOB) {
+            const c2<R> h = *elem(H, g, 0, kx, r.ky, r.kz);
+            const double C = (u.dt * k.s1) * A;
+            re += -C * (double)h.y;
+            im += C * (double)h.x;
+          }
+          sn = mk<R>((R)re, (R)im);
+        } else {
+          const double A = 1.0 / (1.0 - ((u.L * u.kphi) * u.dt) * k.k2);
+          const double B = -(u.L * u.dt) * A;
+          sn = mk<R>((R)(A * (double)sn.x + B * (double)w.x), (R)(A * (double)sn.y + B * (double)w.y));
+        }
+        if (DC64 && kx == 0 && r.ky == 0 && r.kz == 0) {
+          if (r.f != 0) dc64[r.f] = dc64[r.f] - (u.L * u.dt) * (double)w.x;
+          sn = mk<R>((R)dc64[r.f], R(0));
+        }
+        *sp = sn;
+        *elem(W, g, r.f, kx, r.ky, r.kz) = sn;
+      }
+      if (PARSEVAL) {
+        const double wgt = (kx == 0 || kx == g.M) ? 1.0 : 2.0;
+        const double m2 = (double)sn.x * (double)sn.x + (double)sn.y * (double)sn.y;
+        acc += (0.5 * kap) * wgt * k.kp2 * m2;
+      }
+    }
+  }
+  if (PARSEVAL) {
+    const double s = block_sum<NB>(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s * (double)g.P;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Final diagnostics reduction (one block): Neumaier-compensated sums over the
+// block partials in fixed order, extrema with the reference's initial values.
+// ---------------------------------------------------------------------------
+struct DiagSrc {
+  const double* nl; long n_nl;  // NPART_NL per block
+  const double* fi; long n_fi;  // one per block
+  const double* cmean;          // device pointer to mean(c), or null
+  int cmean_is_float;
+  const float* cmean_f;
+};
+__device__ __forceinline__ void neum(double& s, double& c, double x) {
+  const double t = s + x;
+  if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+  s = t;
+}
+__global__ void __launch_bounds__(NB) k_diag_final(DiagSrc d, double* __restrict__ out) {
+  __shared__ double red[NB];
+  double sch = 0, cch = 0, set = 0, cet = 0, sfi = 0, cfi = 0;
+  double cmax = 0.0, cmin = 1.0, pmin = 1.0, p3 = 0.0;
+  for (long b = threadIdx.x; b < d.n_nl; b += NB) {
+    const double* o = d.nl + b * NPART_NL;
+    neum(sch, cch, o[0]);
+    neum(set, cet, o[1]);
+    if (o[2] > cmax) cmax = o[2];
+    if (o[3] <= cmin) cmin = o[3];
+    if (o[4] < pmin) pmin = o[4];
+    if (o[5] > p3) p3 = o[5];
+  }
+  for (long b = threadIdx.x; b < d.n_fi; b += NB) neum(sfi, cfi, d.fi[b]);
+  const double fch = block_sum<NB>(sch + cch, red);
+  const double feta = block_sum<NB>(set + cet, red);
+  const double fint = block_sum<NB>(sfi + cfi, red);
+  const double e0 = block_ext<NB, true>(cmax, red), e1 = block_ext<NB, false>(cmin, red);
+  const double e2 = block_ext<NB, false>(pmin, red), e3 = block_ext<NB, true>(p3, red);
+  if (threadIdx.x == 0) {
+    out[0] = fint; out[1] = fch; out[2] = feta; out[3] = (fint + fch) + feta;
+    out[4] = e0; out[5] = e1; out[6] = e2; out[7] = e3;
+    out[8] = d.cmean_is_float ? (double)*d.cmean_f : (d.cmean ? *d.cmean : 0.0);
+  }
+}
+
+}  // namespace pf
+
+// ===========================================================================
+// Handle
+// ===========================================================================
+using namespace pf;
+
+struct pf_handle {
+  Geo g{};
+  int mode = 0;
+  pf_params p{};
+  UCoef u{};
+  hipStream_t s = nullptr;
+  int dev = 0;
+  long step = 0;  // completed steps
+
+  double* dk = nullptr;
+  KTab kt{};
+  double* dgam = nullptr;
+
+  // faithful (FP64) banks
+  Bank<double> fs{}, fgrad{}, fdfdc{}, fdum{}, fpsi{};
+  double* fmob = nullptr;
+  // reduced banks (precision R, stored untyped)
+  void *S_main = nullptr, *S_nyq = nullptr, *W_main = nullptr, *W_nyq = nullptr;
+  void *G_main = nullptr, *G_nyq = nullptr, *mob = nullptr;
+  double* dc64 = nullptr;
+
+  double* part_nl = nullptr;
+  double* part_fi = nullptr;
+  long n_part_nl = 0, n_part_fi = 0;
+  bool fint_ready = false;  // part_fi holds f_int of the current state
+  double* ddiag = nullptr;  // 9 doubles per record
+  long ddiag_cap = 0;
+
+  double* staging = nullptr;  // (var+1)*P doubles for state I/O
+
+  std::vector<void*> allocs;
+
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_stage;
+  size_t ev_used = 0;
+  double st_ms[NSTAGE] = {};
+  long st_cnt[NSTAGE] = {};
+};
+
+namespace {
+
+int dalloc(pf_handle* h, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(PF_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+  }
+  h->allocs.push_back(*p);
+  return 0;
+}
+template <typename R> int alloc_bank(pf_handle* h, Bank<R>& b, int nf) {
+  int rc = dalloc(h, (void**)&b.main, sizeof(c2<R>) * (size_t)h->g.hmain * nf);
+  if (!rc) rc = dalloc(h, (void**)&b.nyq, sizeof(c2<R>) * (size_t)h->g.plane * nf);
+  return rc;
+}
+template <typename R> Bank<R> sub(const Bank<R>& b, const Geo& g, int f) {
+  return Bank<R>{b.main + (long)f * g.hmain, b.nyq + (long)f * g.plane};
+}
+
+// ---- timers ----
+void tbeg(pf_handle* h, int stage) {
+  if (!h->timing) return;
+  if (h->ev_used + 2 > h->ev.size()) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) { h->timing = false; return; }
+      h->ev.push_back(e);
+    }
+    h->ev_stage.resize(h->ev.size());
+  }
+  h->ev_stage[h->ev_used] = stage;
+  (void)hipEventRecord(h->ev[h->ev_used], h->s);
+}
+void tend(pf_handle* h) {
+  if (!h->timing) return;
+  (void)hipEventRecord(h->ev[h->ev_used + 1], h->s);
+  h->ev_used += 2;
+}
+void tflush(pf_handle* h) {
+  if (h->ev_used == 0) return;
+  (void)hipEventSynchronize(h->ev[h->ev_used - 1]);
+  for (size_t i = 0; i < h->ev_used; i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]) == hipSuccess) {
+      h->st_ms[h->ev_stage[i]] += ms;
+      h->st_cnt[h->ev_stage[i]] += 1;
+    }
+  }
+  h->ev_used = 0;
+}
+
+unsigned grid_for(long n, long per) { return (unsigned)std::min<long>(kMaxGrid, std::max<long>(1, (n + per - 1) / per)); }
+unsigned row_block(const Geo& g) {
+  int b = ((g.M + 1 + 63) / 64) * 64;
+  return (unsigned)std::min(b, NB);
+}
+
+// Final diag reduction into ddiag record `idx`.
+int diag_final(pf_handle* h, long idx, bool with_nl, bool with_fi, const double* cmean, const float* cmean_f) {
+  DiagSrc d{};
+  d.nl = h->part_nl; d.n_nl = with_nl ? h->n_part_nl : 0;
+  d.fi = h->part_fi; d.n_fi = with_fi ? h->n_part_fi : 0;
+  d.cmean = cmean;
+  d.cmean_f = cmean_f;
+  d.cmean_is_float = cmean_f != nullptr;
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(NB), 0, h->s, d, h->ddiag + 9 * idx);
+  return 0;
+}
+
+int ensure_diag(pf_handle* h, long n) {
+  if (n <= h->ddiag_cap) return 0;
+  if (h->ddiag) {
+    (void)hipFree(h->ddiag);
+    h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void*)h->ddiag), h->allocs.end());
+  }
+  h->ddiag = nullptr;
+  h->ddiag_cap = 0;
+  int rc = dalloc(h, (void**)&h->ddiag, sizeof(double) * 9 * (size_t)n);
+  if (!rc) h->ddiag_cap = n;
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Faithful step (FP64): the reference's loop sequence.
+// ---------------------------------------------------------------------------
+int step_faithful(pf_handle* h, bool diag, long rec) {
+  const Geo& g = h->g;
+  hipStream_t s = h->s;
+  const unsigned rb = row_block(g);
+  int rc;
+  double* st = (double*)h->fs.main;
+  // loops 5-10 (read the real state before it is transformed in place)
+  tbeg(h, ST_NONLINEAR);
+  const size_t shb = sizeof(double) * ((size_t)g.var * NB + NB);
+  if (diag)
+    hipLaunchKernelGGL((k_nl_faithful<true>), dim3(grid_for(g.P, NB)), dim3(NB), shb, s, g, h->u, h->dgam,
+                       st, (double*)h->fdfdc.main, (double*)h->fpsi.main, h->fmob, h->part_nl);
+  else
+    hipLaunchKernelGGL((k_nl_faithful<false>), dim3(grid_for(g.P, NB)), dim3(NB), shb, s, g, h->u,
+                       h->dgam, st, (double*)h->fdfdc.main, (double*)h->fpsi.main, h->fmob, h->part_nl);
+  tend(h);
+  // loops 1-4: F(phi_i), F(c); gradients; f_int
+  tbeg(h, ST_X_R2C);
+  if ((rc = fwd3d<double>(h->fs.main, h->fs.nyq, g.nx, g.ny, g.nz, g.F, 1.0 / (double)(float)g.P, s))) return rc;
+  tend(h);
+  tbeg(h, ST_GRAD);
+  hipLaunchKernelGGL(k_grad_spec, dim3(grid_for((long)g.plane * g.F, 1)), dim3(rb), 0, s, g, h->kt, h->fs,
+                     h->fgrad, (long)g.plane * g.F);
+  if ((rc = inv3d<double>(h->fgrad.main, h->fgrad.nyq, g.nx, g.ny, g.nz, 3 * g.F, s))) return rc;
+  hipLaunchKernelGGL(k_fint_faithful, dim3(grid_for((long)g.F * g.P, NB)), dim3(NB), 0, s, g, h->u,
+                     (double*)h->fgrad.main, h->part_fi);
+  h->n_part_fi = grid_for((long)g.F * g.P, NB);
+  tend(h);
+  if (diag) {
+    tbeg(h, ST_DIAG);
+    diag_final(h, rec, true, true, (const double*)h->fs.main, nullptr);  // Re F(c)(0) = mean(c)
+    tend(h);
+  }
+  // loop 10: F(df_dc)  (F(c) of loop 10 is the same transform as loop 3's; reused)
+  tbeg(h, ST_MOBILITY);
+  if ((rc = fwd3d<double>(h->fdfdc.main, h->fdfdc.nyq, g.nx, g.ny, g.nz, 1, 1.0 / (double)(float)g.P, s))) return rc;
+  // loop 11
+  hipLaunchKernelGGL((k_ghat<double>), dim3(grid_for(g.plane, 1)), dim3(rb), 0, s, g, h->u, h->kt, h->fdfdc, 0,
+                     h->fs, 0, h->fdum, (long)g.plane);
+  if ((rc = inv3d<double>(h->fdum.main, h->fdum.nyq, g.nx, g.ny, g.nz, 1, s))) return rc;
+  // loop 12
+  hipLaunchKernelGGL(k_mult_faithful, dim3(grid_for(g.P, NB)), dim3(NB), 0, s, g.P, h->u.D,
+                     (double*)h->fdum.main, h->fmob);
+  if ((rc = fwd3d<double>(h->fdum.main, h->fdum.nyq, g.nx, g.ny, g.nz, 1, 1.0 / (double)(float)g.P, s))) return rc;
+  tend(h);
+  // loop 13
+  tbeg(h, ST_UPDATE);
+  hipLaunchKernelGGL(k_ch_faithful, dim3(grid_for(g.plane, 1)), dim3(rb), 0, s, g, h->u, h->kt, h->fs, h->fdum,
+                     h->fdfdc, (long)g.plane);
+  tend(h);
+  // loop 14: F(df_dphi_i) (F(phi_i) is loop 1's transform; reused)
+  tbeg(h, ST_Y_FWD);
+  if ((rc = fwd3d<double>(h->fpsi.main, h->fpsi.nyq, g.nx, g.ny, g.nz, g.var, 1.0 / (double)(float)g.P, s))) return rc;
+  tend(h);
+  tbeg(h, ST_UPDATE);
+  hipLaunchKernelGGL(k_ac_faithful, dim3(grid_for((long)g.plane * g.var, 1)), dim3(rb), 0, s, g, h->u, h->kt,
+                     h->fs, h->fpsi, (long)g.plane * g.var);
+  tend(h);
+  // inverse transforms of c (loop 13) and phi_i (loop 14), batched
+  tbeg(h, ST_Z_INV);
+  if ((rc = inv3d<double>(h->fs.main, h->fs.nyq, g.nx, g.ny, g.nz, g.F, s))) return rc;
+  tend(h);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Reduced step: W holds the real state on entry and on exit; S the spectrum.
+// ---------------------------------------------------------------------------
+template <typename R>
+int step_reduced(pf_handle* h, bool diag, long rec, bool fint_next) {
+  const Geo& g = h->g;
+  hipStream_t s = h->s;
+  const bool mob = h->p.D_gb != 0.0;
+  constexpr bool DC = sizeof(R) == 4;
+  Bank<R> S{(c2<R>*)h->S_main, (c2<R>*)h->S_nyq}, W{(c2<R>*)h->W_main, (c2<R>*)h->W_nyq};
+  Bank<R> G{(c2<R>*)h->G_main, (c2<R>*)h->G_nyq};
+  const R invP = (R)(1.0 / (double)(float)g.P);
+  const unsigned rb = row_block(g);
+  int rc;
+
+  if (diag && !h->fint_ready) {
+    tbeg(h, ST_DIAG);
+    const long rows = (long)g.plane * g.F;
+    hipLaunchKernelGGL((k_update<R, false, false, true, false>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
+                       h->u, h->kt, S, W, G, h->dc64, h->part_fi, rows);
+    h->n_part_fi = grid_for(rows, 1);
+    tend(h);
+  }
+  tbeg(h, ST_NONLINEAR);
+  {
+    const size_t shb = sizeof(R) * (size_t)g.var * NB + 16 + sizeof(double) * NB;
+    R* wr = (R*)h->W_main;
+    R* md = (R*)h->mob;
+    const dim3 gr(grid_for(g.P, NB));
+#define PF_NL(D, M) hipLaunchKernelGGL((k_nl_reduced<R, D, M>), gr, dim3(NB), shb, s, g, h->u, wr, md, h->part_nl)
+    if (diag) { if (mob) PF_NL(true, true); else PF_NL(true, false); }
+    else { if (mob) PF_NL(false, true); else PF_NL(false, false); }
+#undef PF_NL
+  }
+  tend(h);
+  if (diag) {
+    tbeg(h, ST_DIAG);
+    if (DC) diag_final(h, rec, true, true, h->dc64, nullptr);
+    else diag_final(h, rec, true, true, (const double*)h->S_main, nullptr);
+    tend(h);
+  }
+  // forward transforms of df_dc, df_dphi (with f_trans normalisation)
+  const long M = g.M;
+  tbeg(h, ST_X_R2C);
+  if ((rc = x_r2c<R>(g.nx, W.main, W.nyq, (long)g.ny * g.nz * g.F, invP, s))) return rc;
+  tend(h);
+  tbeg(h, ST_Y_FWD);
+  if ((rc = col_pass<R>(g.ny, false, W.main, M, (long)g.nz * g.F, s))) return rc;
+  if ((rc = row_pass<R>(g.ny, false, W.nyq, (long)g.nz * g.F, s))) return rc;
+  tend(h);
+  tbeg(h, ST_Z_FWD);
+  if ((rc = col_pass<R>(g.nz, false, W.main, M * g.ny, g.F, s))) return rc;
+  if ((rc = col_pass<R>(g.nz, false, W.nyq, g.ny, g.F, s))) return rc;
+  tend(h);
+  if (mob) {
+    // variable-mobility correction (loops 11-12) for c
+    tbeg(h, ST_MOBILITY);
+    hipLaunchKernelGGL((k_ghat<R>), dim3(grid_for(g.plane, 1)), dim3(rb), 0, s, g, h->u, h->kt, W, 0, S, 0, G,
+                       (long)g.plane);
+    if ((rc = inv3d<R>(G.main, G.nyq, g.nx, g.ny, g.nz, 1, s))) return rc;
+    hipLaunchKernelGGL((k_mult<R>), dim3(grid_for(g.P, NB)), dim3(NB), 0, s, g.P, (R*)G.main, (const R*)h->mob);
+    if ((rc = fwd3d<R>(G.main, G.nyq, g.nx, g.ny, g.nz, 1, invP, s))) return rc;
+    tend(h);
+  }
+  tbeg(h, ST_UPDATE);
+  {
+    const long rows = (long)g.plane * g.F;
+    const dim3 gr(grid_for(rows, 1));
+#define PF_UP(MB, PV) \
+  hipLaunchKernelGGL((k_update<R, DC, MB, PV, true>), gr, dim3(NB), 0, s, g, h->u, h->kt, S, W, G, h->dc64, h->part_fi, rows)
+    if (mob) { if (fint_next) PF_UP(true, true); else PF_UP(true, false); }
+    else { if (fint_next) PF_UP(false, true); else PF_UP(false, false); }
+#undef PF_UP
+    if (fint_next) h->n_part_fi = gr.x;
+    h->fint_ready = fint_next;
+  }
+  tend(h);
+  tbeg(h, ST_Z_INV);
+  if ((rc = col_pass<R>(g.nz, true, W.main, M * g.ny, g.F, s))) return rc;
+  if ((rc = col_pass<R>(g.nz, true, W.nyq, g.ny, g.F, s))) return rc;
+  tend(h);
+  tbeg(h, ST_Y_INV);
+  if ((rc = col_pass<R>(g.ny, true, W.main, M, (long)g.nz * g.F, s))) return rc;
+  if ((rc = row_pass<R>(g.ny, true, W.nyq, (long)g.nz * g.F, s))) return rc;
+  tend(h);
+  tbeg(h, ST_X_C2R);
+  if ((rc = x_c2r<R>(g.nx, W.main, W.nyq, (long)g.ny * g.nz * g.F, s))) return rc;
+  tend(h);
+  return 0;
+}
+
+template <typename R> int alloc_reduced(pf_handle* h) {
+  Bank<R> S{}, W{}, G{};
+  int rc = alloc_bank<R>(h, S, h->g.F);
+  if (!rc) rc = alloc_bank<R>(h, W, h->g.F);
+  if (!rc && h->p.D_gb != 0.0) {
+    rc = alloc_bank<R>(h, G, 1);
+    if (!rc) rc = dalloc(h, &h->mob, sizeof(R) * (size_t)h->g.P);
+  }
+  h->S_main = S.main; h->S_nyq = S.nyq; h->W_main = W.main; h->W_nyq = W.nyq;
+  h->G_main = G.main; h->G_nyq = G.nyq;
+  if (!rc) rc = dalloc(h, (void**)&h->dc64, sizeof(double) * h->g.F);
+  return rc;
+}
+
+template <typename R> int put_reduced(pf_handle* h, const double* phi, const double* con) {
+  const Geo& g = h->g;
+  // W <- real state; S <- F(W) with f_trans normalisation
+  hipLaunchKernelGGL((k_scatter_state<R>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s, h->staging,
+                     h->staging + (size_t)g.var * g.P, (R*)h->W_main, g.P, g.var);
+  PF_HIP(hipMemcpyAsync(h->S_main, h->W_main, sizeof(R) * (size_t)g.P * g.F, hipMemcpyDeviceToDevice, h->s));
+  int rc = fwd3d<R>((c2<R>*)h->S_main, (c2<R>*)h->S_nyq, g.nx, g.ny, g.nz, g.F, (R)(1.0 / (double)(float)g.P), h->s);
+  if (rc) return rc;
+  // FP64 means (k = 0 modes), Neumaier-compensated
+  std::vector<double> dc(g.F, 0.0);
+  for (int f = 0; f < g.F; ++f) {
+    double sm = 0.0, cc = 0.0;
+    for (long t = 0; t < g.P; ++t) {
+      const double x = (f == 0) ? con[t] : phi[t * g.var + (f - 1)];
+      const double tt = sm + x;
+      if (std::fabs(sm) >= std::fabs(x)) cc += (sm - tt) + x; else cc += (x - tt) + sm;
+      sm = tt;
+    }
+    dc[f] = (sm + cc) / (double)(float)g.P;
+  }
+  PF_HIP(hipMemcpyAsync(h->dc64, dc.data(), sizeof(double) * g.F, hipMemcpyHostToDevice, h->s));
+  PF_HIP(hipStreamSynchronize(h->s));
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" int pf_create(const pf_params* p, int nx, int ny, int nz, int var, int mode, pf_handle** out) {
+  if (!p || !out) return fail(PF_EINVAL, "null argument");
+  *out = nullptr;
+  if (!grid_size_supported(nx) || !grid_size_supported(ny) || !grid_size_supported(nz))
+    return fail(PF_EINVAL, std::string("unsupported grid; each axis must be one of ") + supported_sizes_text());
+  if (var < 1 || var > 256) return fail(PF_EINVAL, "var must be in [1, 256]");
+  if (mode < PF_MODE_FP64_FAITHFUL || mode > PF_MODE_FP32_MIXED) return fail(PF_EINVAL, "bad mode");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PF_EHIP, "no HIP device available");
+  pf_handle* h = new pf_handle();
+  h->mode = mode;
+  h->p = *p;
+  Geo& g = h->g;
+  g.nx = nx; g.ny = ny; g.nz = nz; g.M = nx / 2; g.var = var; g.F = var + 1;
+  g.P = (long)nx * ny * nz; g.plane = (long)ny * nz; g.hmain = (long)g.M * ny * nz;
+  UCoef& u = h->u;
+  u.dt = p->t_step; u.kphi = p->grad_coeff_phi; u.kc = p->grad_coeff_c; u.D = p->D_L; u.L = p->mob_phi;
+  u.Dgb = p->D_gb; u.am = p->am; u.ap = p->ap; u.cm = p->con_0_mat; u.cp = p->con_0_ppt;
+  u.s1g = p->sigma_1; u.s2g = p->sigma_2; u.invP = 1.0 / (double)(float)g.P;
+  int rc = 0;
+  auto bail = [&](int code) { pf_destroy(h); return code; };
+  if (hipGetDevice(&h->dev) != hipSuccess) return bail(fail(PF_EHIP, "hipGetDevice failed"));
+  if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(PF_EHIP, "stream creation failed"));
+  // wavenumber tables
+  {
+    const int M = g.M;
+    std::vector<double> tab(2 * (M + 1) + 2 * ny + 2 * nz);
+    double* q = tab.data();
+    for (int i = 0; i <= M; ++i) { q[i] = kaxis(i, nx, false); q[M + 1 + i] = kaxis(i, nx, true); }
+    q += 2 * (M + 1);
+    for (int i = 0; i < ny; ++i) { q[i] = kaxis(i, ny, false); q[ny + i] = kaxis(i, ny, true); }
+    q += 2 * ny;
+    for (int i = 0; i < nz; ++i) { q[i] = kaxis(i, nz, false); q[nz + i] = kaxis(i, nz, true); }
+    if ((rc = dalloc(h, (void**)&h->dk, sizeof(double) * tab.size()))) return bail(rc);
+    if (hipMemcpy(h->dk, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(PF_EHIP, "copy of wavenumber tables failed"));
+    const double* d = h->dk;
+    h->kt.ax = d; h->kt.axp = d + M + 1; d += 2 * (M + 1);
+    h->kt.ay = d; h->kt.ayp = d + ny; d += 2 * ny;
+    h->kt.az = d; h->kt.azp = d + nz;
+  }
+  if (mode == PF_MODE_FP64_FAITHFUL) {
+    std::vector<double> gam((size_t)var * var);
+    pf_gamma(var, p->sigma_1, p->sigma_2, gam.data());
+    if ((rc = dalloc(h, (void**)&h->dgam, sizeof(double) * gam.size()))) return bail(rc);
+    if (hipMemcpy(h->dgam, gam.data(), sizeof(double) * gam.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(PF_EHIP, "copy of gamma failed"));
+    if ((rc = alloc_bank<double>(h, h->fs, g.F)) || (rc = alloc_bank<double>(h, h->fgrad, 3 * g.F)) ||
+        (rc = alloc_bank<double>(h, h->fdfdc, 1)) || (rc = alloc_bank<double>(h, h->fdum, 1)) ||
+        (rc = alloc_bank<double>(h, h->fpsi, var)) || (rc = dalloc(h, (void**)&h->fmob, sizeof(double) * g.P)))
+      return bail(rc);
+  } else if (mode == PF_MODE_FP64_REDUCED) {
+    if ((rc = alloc_reduced<double>(h))) return bail(rc);
+  } else {
+    if ((rc = alloc_reduced<float>(h))) return bail(rc);
+  }
+  if ((rc = dalloc(h, (void**)&h->part_nl, sizeof(double) * NPART_NL * kMaxGrid))) return bail(rc);
+  if ((rc = dalloc(h, (void**)&h->part_fi, sizeof(double) * kMaxGrid))) return bail(rc);
+  if ((rc = dalloc(h, (void**)&h->staging, sizeof(double) * (size_t)g.P * g.F))) return bail(rc);
+  h->n_part_nl = grid_for(g.P, NB);
+  if ((rc = ensure_diag(h, 64))) return bail(rc);
+  *out = h;
+  return 0;
+}
+
+extern "C" int pf_destroy(pf_handle* h) {
+  if (!h) return 0;
+  if (h->s) (void)hipStreamSynchronize(h->s);
+  for (void* p : h->allocs) (void)hipFree(p);
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  if (h->s) (void)hipStreamDestroy(h->s);
+  delete h;
+  return 0;
+}
+
+extern "C" int pf_put_state(pf_handle* h, const double* phi, const double* con) {
+  if (!h || !phi || !con) return fail(PF_EINVAL, "null argument");
+  const Geo& g = h->g;
+  PF_HIP(hipMemcpyAsync(h->staging, phi, sizeof(double) * (size_t)g.P * g.var, hipMemcpyHostToDevice, h->s));
+  PF_HIP(hipMemcpyAsync(h->staging + (size_t)g.P * g.var, con, sizeof(double) * (size_t)g.P,
+                        hipMemcpyHostToDevice, h->s));
+  h->fint_ready = false;
+  if (h->mode == PF_MODE_FP64_FAITHFUL) {
+    hipLaunchKernelGGL((k_scatter_state<double>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s, h->staging,
+                       h->staging + (size_t)g.var * g.P, (double*)h->fs.main, g.P, g.var);
+    PF_HIP(hipStreamSynchronize(h->s));
+    return 0;
+  }
+  if (h->mode == PF_MODE_FP64_REDUCED) return put_reduced<double>(h, phi, con);
+  return put_reduced<float>(h, phi, con);
+}
+
+extern "C" int pf_get_state(pf_handle* h, double* phi, double* con) {
+  if (!h || !phi || !con) return fail(PF_EINVAL, "null argument");
+  const Geo& g = h->g;
+  double* sphi = h->staging;
+  double* scon = h->staging + (size_t)g.var * g.P;
+  if (h->mode == PF_MODE_FP64_FAITHFUL)
+    hipLaunchKernelGGL((k_gather_state<double>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s,
+                       (const double*)h->fs.main, sphi, scon, g.P, g.var);
+  else if (h->mode == PF_MODE_FP64_REDUCED)
+    hipLaunchKernelGGL((k_gather_state<double>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s,
+                       (const double*)h->W_main, sphi, scon, g.P, g.var);
+  else
+    hipLaunchKernelGGL((k_gather_state<float>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s,
+                       (const float*)h->W_main, sphi, scon, g.P, g.var);
+  PF_HIP(hipGetLastError());
+  PF_HIP(hipMemcpyAsync(phi, sphi, sizeof(double) * (size_t)g.P * g.var, hipMemcpyDeviceToHost, h->s));
+  PF_HIP(hipMemcpyAsync(con, scon, sizeof(double) * (size_t)g.P, hipMemcpyDeviceToHost, h->s));
+  PF_HIP(hipStreamSynchronize(h->s));
+  return 0;
+}
+
+extern "C" int pf_step(pf_handle* h, int nsteps, int diag_every, pf_diag* out) {
+  if (!h) return fail(PF_EINVAL, "null handle");
+  if (nsteps < 0) return fail(PF_EINVAL, "nsteps < 0");
+  int rc = 0;
+  if (out && (rc = ensure_diag(h, std::max(1, nsteps)))) return rc;
+  std::vector<char> had(nsteps, 0);
+  for (int i = 0; i < nsteps; ++i) {
+    const long stepno = h->step + 1;
+    const bool diag = out && diag_every > 0 && (stepno % diag_every) == 0;
+    const bool next_diag = out && diag_every > 0 && ((stepno + 1) % diag_every) == 0 && (i + 1) < nsteps;
+    if (h->mode == PF_MODE_FP64_FAITHFUL) rc = step_faithful(h, diag, i);
+    else if (h->mode == PF_MODE_FP64_REDUCED) rc = step_reduced<double>(h, diag, i, next_diag);
+    else rc = step_reduced<float>(h, diag, i, next_diag);
+    if (rc) return rc;
+    had[i] = diag;
+    h->step = stepno;
+  }
+  PF_HIP(hipGetLastError());
+  std::vector<double> hd;
+  if (out && nsteps > 0) {
+    hd.resize(9 * (size_t)nsteps);
+    PF_HIP(hipMemcpyAsync(hd.data(), h->ddiag, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, h->s));
+  }
+  PF_HIP(hipStreamSynchronize(h->s));
+  tflush(h);
+  if (out) {
+    bool bad = false;
+    for (int i = 0; i < nsteps; ++i) {
+      pf_diag& d = out[i];
+      d.step = (int)(h->step - nsteps + 1 + i);
+      d.valid = had[i];
+      const double* r = hd.data() + 9 * (size_t)i;
+      if (had[i]) {
+        d.f_int = r[0]; d.f_ch = r[1]; d.f_eta = r[2]; d.e_tot = r[3];
+        d.c_max = r[4]; d.c_min = r[5]; d.phi_min = r[6]; d.phi_3_max = r[7]; d.c_mean = r[8];
+        if (!std::isfinite(d.e_tot) || !std::isfinite(d.c_mean)) bad = true;
+      } else {
+        d.f_int = d.f_ch = d.f_eta = d.e_tot = d.c_max = d.c_min = d.phi_min = d.phi_3_max = d.c_mean = NAN;
+      }
+    }
+    if (bad) return fail(PF_ENONFINITE, "non-finite energy or mass in diagnostics");
+  }
+  return 0;
+}
+
+extern "C" int pf_diagnose(pf_handle* h, pf_diag* out) {
+  if (!h || !out) return fail(PF_EINVAL, "null argument");
+  const Geo& g = h->g;
+  hipStream_t s = h->s;
+  int rc = ensure_diag(h, 1);
+  if (rc) return rc;
+  const unsigned rb = row_block(g);
+  const long rows = (long)g.plane * g.F;
+  const unsigned gr_nl = grid_for(g.P, NB);
+  if (h->mode == PF_MODE_FP64_FAITHFUL) {
+    const size_t shb = sizeof(double) * ((size_t)g.var * NB + NB);
+    hipLaunchKernelGGL((k_nl_faithful<true>), dim3(gr_nl), dim3(NB), shb, s, g, h->u, h->dgam,
+                       (const double*)h->fs.main, (double*)h->fdfdc.main, (double*)h->fpsi.main, h->fmob,
+                       h->part_nl);
+    // f_int by Parseval on a transformed copy of the state (same quantity as loops 1-4)
+    PF_HIP(hipMemcpyAsync(h->fgrad.main, h->fs.main, sizeof(double) * (size_t)g.P * g.F,
+                          hipMemcpyDeviceToDevice, s));
+    if ((rc = fwd3d<double>(h->fgrad.main, h->fgrad.nyq, g.nx, g.ny, g.nz, g.F, 1.0 / (double)(float)g.P, s)))
+      return rc;
+    hipLaunchKernelGGL((k_update<double, false, false, true, false>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
+                       h->u, h->kt, h->fgrad, h->fgrad, h->fgrad, h->dc64, h->part_fi, rows);
+    h->n_part_fi = grid_for(rows, 1);
+    diag_final(h, 0, true, true, (const double*)h->fgrad.main, nullptr);
+  } else {
+    const bool f32 = h->mode == PF_MODE_FP32_MIXED;
+    if (f32) {
+      Bank<float> S{(c2<float>*)h->S_main, (c2<float>*)h->S_nyq};
+      hipLaunchKernelGGL((k_update<float, false, false, true, false>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
+                         h->u, h->kt, S, S, S, h->dc64, h->part_fi, rows);
+      const size_t shb = sizeof(float) * (size_t)g.var * NB + 16 + sizeof(double) * NB;
+      hipLaunchKernelGGL((k_nl_reduced<float, true, false, false>), dim3(gr_nl), dim3(NB), shb, s, g, h->u,
+                         (float*)h->W_main, (float*)nullptr, h->part_nl);
+    } else {
+      Bank<double> S{(c2<double>*)h->S_main, (c2<double>*)h->S_nyq};
+      hipLaunchKernelGGL((k_update<double, false, false, true, false>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
+                         h->u, h->kt, S, S, S, h->dc64, h->part_fi, rows);
+      const size_t shb = sizeof(double) * (size_t)g.var * NB + 16 + sizeof(double) * NB;
+      hipLaunchKernelGGL((k_nl_reduced<double, true, false, false>), dim3(gr_nl), dim3(NB), shb, s, g, h->u,
+                         (double*)h->W_main, (double*)nullptr, h->part_nl);
+    }
+    h->n_part_fi = grid_for(rows, 1);
+    h->fint_ready = true;
+    diag_final(h, 0, true, true, f32 ? h->dc64 : (const double*)h->S_main, nullptr);
+  }
+  PF_HIP(hipGetLastError());
+  double r[9];
+  PF_HIP(hipMemcpyAsync(r, h->ddiag, sizeof(r), hipMemcpyDeviceToHost, s));
+  PF_HIP(hipStreamSynchronize(s));
+  out->step = (int)(h->step + 1);
+  out->valid = 1;
+  out->f_int = r[0]; out->f_ch = r[1]; out->f_eta = r[2]; out->e_tot = r[3];
+  out->c_max = r[4]; out->c_min = r[5]; out->phi_min = r[6]; out->phi_3_max = r[7]; out->c_mean = r[8];
+  if (!std::isfinite(out->e_tot) || !std::isfinite(out->c_mean))
+    return fail(PF_ENONFINITE, "non-finite energy or mass in diagnostics");
+  return 0;
+}
+
+extern "C" int pf_set_timing(pf_handle* h, int on) {
+  if (!h) return fail(PF_EINVAL, "null handle");
+  h->timing = on != 0;
+  for (int i = 0; i < NSTAGE; ++i) { h->st_ms[i] = 0.0; h->st_cnt[i] = 0; }
+  return 0;
+}
+
+extern "C" int pf_stage_times(pf_handle* h, double* ms, long* count, int n) {
+  if (!h) return fail(PF_EINVAL, "null handle");
+  for (int i = 0; i < n && i < NSTAGE; ++i) {
+    if (ms) ms[i] = h->st_ms[i];
+    if (count) count[i] = h->st_cnt[i];
+  }
+  return NSTAGE;
+}
+
+extern "C" int pf_stage_name(int stage, char* buf, int len) {
+  if (stage < 0 || stage >= NSTAGE || !buf || len < 1) return fail(PF_EINVAL, "bad stage");
+  std::snprintf(buf, (size_t)len, "%s", kStageNames[stage]);
+  return 0;
+}
+
+extern "C" double pf_stage_bytes(pf_handle* h, int stage) {
+  if (!h || stage < 0 || stage >= NSTAGE) return 0.0;
+  const Geo& g = h->g;
+  const double s = (h->mode == PF_MODE_FP32_MIXED) ? 4.0 : 8.0;
+  const double Br = (double)g.P * s;                       // one real field
+  const double Bc = 2.0 * (double)(g.M + 1) * g.plane * s; // one half spectrum
+  const double F = g.F;
+  switch (stage) {
+    case ST_NONLINEAR: return 2.0 * F * Br + (h->p.D_gb != 0.0 ? Br : 0.0);
+    case ST_X_R2C: return F * (Br + Bc);
+    case ST_Y_FWD: case ST_Z_FWD: case ST_Z_INV: case ST_Y_INV: return 2.0 * F * Bc;
+    case ST_X_C2R: return F * (Bc + Br);
+    case ST_UPDATE: return 4.0 * F * Bc;
+    default: return 0.0;
+  }
+}
+
+extern "C" void* pf_stream(pf_handle* h) { return h ? (void*)h->s : nullptr; }

@@ -1,0 +1,96 @@
+// Pointwise (real-space) and k-space kernels of the timestep.
+#pragma once
+#include "pf_fft.hpp"
+
+namespace pf {
+
+// Per-axis wavenumber tables (device, FP64): a_d and the Nyquist-zeroed a'_d
+// for kx in [0, nx/2], ky in [0, ny), kz in [0, nz)   (src/k_space.f90:21-59).
+struct KTab {
+  const double *ax, *axp, *ay, *ayp, *az, *azp;
+};
+
+// Per-point combinations in the reference's evaluation order
+// (src/var_diff.f90:932-935, 978-981, 1019):
+//   kf_sum = i*s1, k_sq = k2, k_cube = i*k3, k_4 = k4;  kp2 = |kf|^2 (Nyquist-zeroed).
+struct KCoef {
+  double s1, k2, k3, k4, kp2;
+};
+__device__ __forceinline__ KCoef kcoef(const KTab& t, int kx, int ky, int kz) {
+  const double ax = t.ax[kx], ay = t.ay[ky], az = t.az[kz];
+  const double axp = t.axp[kx], ayp = t.ayp[ky], azp = t.azp[kz];
+  KCoef k;
+  k.s1 = __dadd_rn(__dadd_rn(axp, ayp), azp);
+  const double x2 = __dmul_rn(ax, ax), y2 = __dmul_rn(ay, ay), z2 = __dmul_rn(az, az);
+  k.k2 = __dadd_rn(__dadd_rn(-x2, -y2), -z2);
+  k.k3 = __dadd_rn(__dadd_rn(-__dmul_rn(axp, x2), -__dmul_rn(ayp, y2)), -__dmul_rn(azp, z2));
+  const double x4 = __dmul_rn(__dmul_rn(x2, ax), ax), y4 = __dmul_rn(__dmul_rn(y2, ay), ay),
+               z4 = __dmul_rn(__dmul_rn(z2, az), az);
+  k.k4 = __dadd_rn(__dadd_rn(x4, y4), z4);
+  k.kp2 = __dadd_rn(__dadd_rn(__dmul_rn(axp, axp), __dmul_rn(ayp, ayp)), __dmul_rn(azp, azp));
+  return k;
+}
+
+// A bank of nf fields in the split-Nyquist layout.
+template <typename R> struct Bank {
+  c2<R>* main;  // nf * M*ny*nz complex (== nf * nx*ny*nz reals)
+  c2<R>* nyq;   // nf * ny*nz complex
+};
+
+// Geometry shared by the kernels.
+struct Geo {
+  int nx, ny, nz, M, var, F;
+  long P, plane, hmain;  // nx*ny*nz, ny*nz, M*ny*nz
+};
+
+// Row decode for k-space kernels: one block per (ky, kz, field) row, threads over
+// kx in [0, M] (kx == M is the Nyquist entry).
+struct RowIdx {
+  int ky, kz, f;
+};
+__device__ __forceinline__ RowIdx row_decode(const Geo& g, long row) {
+  RowIdx r;
+  r.ky = (int)(row % g.ny);
+  const long t = row / g.ny;
+  r.kz = (int)(t % g.nz);
+  r.f = (int)(t / g.nz);
+  return r;
+}
+template <typename R>
+__device__ __forceinline__ c2<R>* elem(const Bank<R>& b, const Geo& g, int f, int kx, int ky, int kz) {
+  if (kx < g.M) return b.main + (long)f * g.hmain + ((long)ky + (long)g.ny * kz) * g.M + kx;
+  return b.nyq + (long)f * g.plane + ky + (long)g.ny * kz;
+}
+
+// Deterministic block reduction helpers (fixed tree order).
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+template <int NT, bool MAX>
+__device__ __forceinline__ double block_ext(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double o = sh[threadIdx.x + s];
+      if (MAX ? (o > sh[threadIdx.x]) : (o < sh[threadIdx.x])) sh[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace pf

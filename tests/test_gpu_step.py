@@ -1,0 +1,57 @@
+"""GPU timestep parity against the CPU FP64 oracle (restatement of
+src/var_diff.f90:474-1118) on seeded inputs small enough for the oracle.
+
+Tolerances: FP64 forms 1e-11 relative per field (max-norm) and 1e-10 relative
+on the summary energies after the stated number of steps; FP32-mixed 1e-4.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+VOR = dict(t_step=0.005, grad_coeff_phi=1.0, grad_coeff_c=1.0)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _compare(pf, oracle, shape, var, nseeds, nsteps, mode, params, tol_state, tol_e):
+    nx, ny, nz = shape
+    phi, con = pf.ic_voronoi(nx, ny, nz, var, nseeds)
+    with pf.Simulation(pf.make_params(**params), nx, ny, nz, var, mode) as sim:
+        sim.put_state(phi, con)
+        dg = sim.step(nsteps, diag_every=2)
+        gphi, gcon = sim.get_state()
+    ophi, ocon = phi.copy(), con.copy()
+    od = oracle.run(oracle.make_params(**params), ophi, ocon, nsteps)
+    print(f"{mode} {shape} var={var} steps={nsteps}: phi {_rel(gphi, ophi):.2e} con {_rel(gcon, ocon):.2e}")
+    assert np.isfinite(gphi).all() and np.isfinite(gcon).all()
+    assert _rel(gphi, ophi) < tol_state
+    assert _rel(gcon, ocon) < tol_state
+    assert len(dg) == nsteps // 2
+    for d in dg:
+        o = od[d["step"] - 1]
+        for k in ("f_int", "f_ch", "f_eta", "e_tot"):
+            assert abs(d[k] - o[k]) <= tol_e * max(abs(o[k]), 1.0), (k, d[k], o[k])
+        for k in ("c_max", "c_min", "phi_min", "phi_3_max"):
+            assert abs(d[k] - o[k]) <= tol_e * 10 + tol_state, (k, d[k], o[k])
+
+
+@pytest.mark.parametrize("shape,var,nseeds", [((16, 16, 16), 3, 6), ((24, 16, 32), 4, 8),
+                                              ((32, 32, 32), 16, 16)])
+@pytest.mark.parametrize("dgb", [0.0, 0.3])
+def test_fp64_faithful_vs_oracle(pf, oracle, shape, var, nseeds, dgb):
+    _compare(pf, oracle, shape, var, nseeds, 10, "fp64_faithful", dict(VOR, D_gb=dgb), 1e-11, 1e-10)
+
+
+@pytest.mark.parametrize("shape,var,nseeds", [((16, 16, 16), 3, 6), ((24, 16, 32), 4, 8),
+                                              ((32, 32, 32), 16, 16)])
+@pytest.mark.parametrize("dgb", [0.0, 0.3])
+def test_fp64_reduced_vs_oracle(pf, oracle, shape, var, nseeds, dgb):
+    _compare(pf, oracle, shape, var, nseeds, 10, "fp64_reduced", dict(VOR, D_gb=dgb), 1e-11, 1e-10)
+
+
+@pytest.mark.parametrize("dgb", [0.0, 0.3])
+def test_fp32_mixed_vs_oracle(pf, oracle, dgb):
+    _compare(pf, oracle, (32, 32, 32), 4, 8, 20, "fp32_mixed", dict(VOR, D_gb=dgb), 1e-4, 1e-4)
