@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark: grid-point-updates/s of the var_diff.f90 timestep on one MI355X.
+
+Workload (the FP32-mixed configuration of the headline metric): a 256^3 periodic
+grid with c + 16 order parameters (17 fields), Voronoi polycrystal initial
+condition with 64 seeds drawn by ran_2 (iseed = -1), c = 0.4 + 0.02(2u-1),
+dt = 0.005, M = L = kappa = 1, free energy from inputs/config.yaml.  A step is one
+pass of the timestep over all 17 fields including the reference's per-step
+diagnostics (summary energies and extrema on every even step).
+
+The line also carries the FP64 forms measured on the same workload, the roofline
+of the dominant kernel (HIP events on the library's stream over the timed
+region) and a CPU baseline: the oracle (a from-scratch FP64 restatement of the
+reference step; the reference binary needs P3DFFT/FFTW/MPI, absent here) timed
+on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-point-updates/s (FP32-mixed vs FP64) at 256³×17 fields; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload_params(pf):
+    return pf.make_params(t_step=0.005, grad_coeff_phi=1.0, grad_coeff_c=1.0, D_S=1.0, D_L=1.0,
+                          D_gb=0.0, mob_phi=1.0, sigma_1=1.5, sigma_2=1.5, am=0.5, ap=5.0,
+                          con_0_mat=0.1, con_0_ppt=1.0)
+
+
+def run_mode(pf, torch, mode, n, var, phi, con, steps, warmup, timing=True):
+    with pf.Simulation(workload_params(pf), n, n, n, var, mode) as sim:
+        sim.put_state(phi, con)
+        sim.step(warmup, diag_every=2)
+        sim.set_timing(timing)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.step(steps, diag_every=2)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        stages = sim.stage_times()
+    return dt, stages
+
+
+def roofline_of(stages, steps):
+    best = max(stages.items(), key=lambda kv: kv[1]["ms"])
+    name, st = best
+    avg_ms = st["ms"] / max(st["count"], 1)
+    achieved = st["bytes"] / (avg_ms * 1e-3) / 1e9 if st["bytes"] > 0 else 0.0
+    return name, avg_ms, achieved, st["bytes"]
+
+
+def cpu_baseline(n, var, phi, con, nsteps):
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    p = oracle.make_params(t_step=0.005, grad_coeff_phi=1.0, grad_coeff_c=1.0, D_gb=0.0)
+    ophi, ocon = phi.copy(), con.copy()
+    t0 = time.perf_counter()
+    oracle.run(p, ophi, ocon, nsteps)
+    dt = time.perf_counter() - t0
+    return dict(value=n ** 3 * nsteps / dt, unit="grid-point-updates/s", cores=threads,
+                kind="port",
+                sample=f"{nsteps} FP64 faithful steps of the same {n}^3 x {var + 1} workload "
+                       f"({dt:.1f} s), oracle/pf_oracle.c with {threads} OpenMP threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--var", type=int, default=16)
+    ap.add_argument("--seeds", type=int, default=64)
+    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU decomposition is a design "
+              "in DESIGN.md, not a measured path.  Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__
+
+    pf = __graft_entry__.load_package()
+    pf.lib()
+    assert torch.cuda.is_available(), "bench.py needs cuda:0"
+    n, var = args.size, args.var
+    log(f"building Voronoi IC {n}^3 x {var + 1} fields, {args.seeds} seeds")
+    phi, con = pf.ic_voronoi(n, n, n, var, args.seeds, -1, 0.4, 0.02)
+
+    log(f"fp32_mixed: {args.warmup} warmup + {args.steps} timed steps")
+    dt, stages = run_mode(pf, torch, "fp32_mixed", n, var, phi, con, args.steps, args.warmup)
+    ms_step = dt / args.steps * 1e3
+    gpups = n ** 3 * args.steps / dt
+    name, avg_ms, achieved, kbytes = roofline_of(stages, args.steps)
+    log(f"fp32_mixed: {ms_step:.3f} ms/step, {gpups:.3e} GPUPS; dominant stage {name} "
+        f"{avg_ms:.3f} ms avg, {achieved:.0f} GB/s")
+    stage_summary = {k: dict(ms_per_step=v["ms"] / max(args.steps, 1),
+                             GBs=(v["bytes"] / (v["ms"] / v["count"] * 1e-3) / 1e9)
+                             if v["count"] and v["bytes"] else None)
+                     for k, v in stages.items() if v["count"]}
+
+    fp64 = {}
+    if not args.no_fp64:
+        for mode in ("fp64_reduced", "fp64_faithful"):
+            log(f"{mode}: 2 warmup + {args.fp64_steps} timed steps")
+            d64, _ = run_mode(pf, torch, mode, n, var, phi, con, args.fp64_steps, 2, timing=False)
+            fp64[mode] = dict(value=n ** 3 * args.fp64_steps / d64, ms_per_step=d64 / args.fp64_steps * 1e3)
+            log(f"{mode}: {fp64[mode]['ms_per_step']:.3f} ms/step")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"cpu baseline: {args.cpu_steps} oracle steps at {n}^3 x {var + 1}")
+        cpu = cpu_baseline(n, var, phi, con, args.cpu_steps)
+        log(f"cpu baseline: {cpu['value']:.3e} GPUPS")
+
+    # whole-step HBM model (reduced, constant mobility): F(4 B_r + 13 B_c) per step
+    F = var + 1
+    Br = n ** 3 * 4.0
+    Bc = 2.0 * (n // 2 + 1) * n * n * 4.0
+    model_bytes = F * (4 * Br + 13 * Bc)
+    out = {
+        "metric": METRIC,
+        "value": gpups,
+        "unit": "grid-point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic: Voronoi polycrystal, {args.seeds} seeds from ran_2(iseed=-1), "
+                f"c = 0.4 + 0.02(2u-1)",
+        "config": {"workload": f"var_diff timestep, {n}^3 grid, c + {var} eta ({F} fields), "
+                               f"FP32-mixed, dt=0.005, M=L=kappa=1, D_gb=0, diagnostics every 2 steps",
+                   "grid": [n, n, n], "var": var, "fields": F, "mode": "fp32_mixed",
+                   "parallelism": "single GPU"},
+        "field_updates_per_s": gpups * F,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
+                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": kbytes},
+        "step_model": {"model_bytes_per_step": model_bytes,
+                       "model_GBs": model_bytes / (ms_step * 1e-3) / 1e9,
+                       "model_frac": model_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "stages": stage_summary,
+        "fp64": fp64,
+        "cpu_baseline": cpu,
+    }
+    if "fp64_faithful" in fp64:
+        out["ratio_vs_fp64_faithful"] = gpups / fp64["fp64_faithful"]["value"]
+    if "fp64_reduced" in fp64:
+        out["ratio_vs_fp64_reduced"] = gpups / fp64["fp64_reduced"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
