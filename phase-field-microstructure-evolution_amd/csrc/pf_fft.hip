@@ -98,7 +98,7 @@ int x_r2c(int nx, c2<R>* data, c2<R>* nyq, long nlines, R scale, hipStream_t s) 
   case N: {                                                                             \
     constexpr int LPB = lines_per_block(N / 2);                                         \
     hipLaunchKernelGGL((k_x_r2c<R, N>), dim3(ceil_div(nlines, LPB)),                    \
-                       dim3(LPB * fft_threads(N / 2)), 0, s, data, nyq, nlines, tw, scale); \
+                       dim3(LPB * plan_T(N / 2)), 0, s, data, nyq, nlines, tw, scale); \
     break;                                                                              \
   }
     PF_GRID_SIZES(PF_CASE)
@@ -117,7 +117,7 @@ int x_c2r(int nx, c2<R>* data, const c2<R>* nyq, long nlines, hipStream_t s) {
   case N: {                                                                          \
     constexpr int LPB = lines_per_block(N / 2);                                      \
     hipLaunchKernelGGL((k_x_c2r<R, N>), dim3(ceil_div(nlines, LPB)),                 \
-                       dim3(LPB * fft_threads(N / 2)), 0, s, data, nyq, nlines, tw); \
+                       dim3(LPB * plan_T(N / 2)), 0, s, data, nyq, nlines, tw); \
     break;                                                                           \
   }
     PF_GRID_SIZES(PF_CASE)
@@ -136,7 +136,7 @@ int col_pass(int n, bool inv, c2<R>* data, long W, long npanels, hipStream_t s) 
   case N: {                                                                                 \
     constexpr int TX = col_tile(N, sizeof(c2<R>));                                          \
     const long tpp = (W + TX - 1) / TX;                                                     \
-    dim3 grid((unsigned)(tpp * npanels)), block(TX * fft_threads(N));                       \
+    dim3 grid((unsigned)(tpp * npanels)), block(TX * plan_T(N));                       \
     if (inv)                                                                                \
       hipLaunchKernelGGL((k_col_fft<R, N, true>), grid, block, 0, s, data, W, tpp, tw);    \
     else                                                                                    \
@@ -158,7 +158,7 @@ int row_pass(int n, bool inv, c2<R>* data, long nlines, hipStream_t s) {
 #define PF_CASE(N)                                                                      \
   case N: {                                                                             \
     constexpr int LPB = lines_per_block(N);                                             \
-    dim3 grid(ceil_div(nlines, LPB)), block(LPB * fft_threads(N));                      \
+    dim3 grid(ceil_div(nlines, LPB)), block(LPB * plan_T(N));                      \
     if (inv)                                                                            \
       hipLaunchKernelGGL((k_row_fft<R, N, true>), grid, block, 0, s, data, nlines, tw);  \
     else                                                                                \

@@ -1,0 +1,367 @@
+// Register-resident mixed-radix Stockham FFT for gfx950.
+//
+// A line of length N is split over T = N / r0 threads.  Pass p (radix r_p,
+// NB_p = N / r_p butterflies, LS_p = r_0 ... r_{p-1}) gives thread t the
+// butterflies j = t + q*T (q < Q_p = ceil(NB_p / T)); butterfly j reads
+// elements j + m*NB_p and writes (j - j%LS_p)*r_p + j%LS_p + m*LS_p.
+// Pass 0 takes its inputs from registers the caller filled (typically straight
+// from HBM) and the last pass leaves its outputs in registers with known element
+// indices, so an N = r0*r1 transform makes a single LDS transpose.  Radices are
+// 16, 12, 10, 8, 6, 5, 4, 3, 2 (largest first; powers of 8 use 8s), so every
+// size this library supports needs at most three passes.
+//
+// The small DFTs are __host__ __device__ so that they, and a sequential
+// emulation of the thread schedule, can be checked on the CPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pf_fft.hpp"
+
+namespace pf {
+
+// ---------------------------------------------------------------------------
+// Plans.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr bool is_pow8(int n) {
+  while (n > 1 && n % 8 == 0) n /= 8;
+  return n == 1;
+}
+__host__ __device__ constexpr int plan_rad(int N, int p) {
+  const int order[9] = {16, 12, 10, 8, 6, 5, 4, 3, 2};
+  int n = N, idx = 0;
+  if (is_pow8(N)) {
+    while (n > 1) {
+      if (idx == p) return 8;
+      n /= 8;
+      ++idx;
+    }
+    return 1;
+  }
+  while (n > 1) {
+    int r = 1;
+    for (int o = 0; o < 9; ++o)
+      if (n % order[o] == 0) { r = order[o]; break; }
+    if (r == 1) return 1;
+    if (idx == p) return r;
+    n /= r;
+    ++idx;
+  }
+  return 1;
+}
+__host__ __device__ constexpr int plan_np(int N) {
+  int p = 0;
+  while (plan_rad(N, p) > 1) ++p;
+  return p;
+}
+__host__ __device__ constexpr int plan_ls(int N, int p) {
+  int ls = 1;
+  for (int q = 0; q < p; ++q) ls *= plan_rad(N, q);
+  return ls;
+}
+__host__ __device__ constexpr int plan_T(int N) { return N / plan_rad(N, 0); }
+__host__ __device__ constexpr int plan_Q(int N, int p) {
+  return (N / plan_rad(N, p) + plan_T(N) - 1) / plan_T(N);
+}
+// registers per thread (complex values)
+__host__ __device__ constexpr int plan_E(int N) {
+  int e = 0;
+  for (int p = 0; p < plan_np(N); ++p) {
+    const int v = plan_Q(N, p) * plan_rad(N, p);
+    if (v > e) e = v;
+  }
+  return e;
+}
+__host__ __device__ constexpr bool plan_ok(int N) {
+  int prod = 1;
+  for (int p = 0; p < plan_np(N); ++p) prod *= plan_rad(N, p);
+  return prod == N && N > 1;
+}
+
+// ---------------------------------------------------------------------------
+// Small DFTs (forward e^{-i}; INV uses conjugate roots), in place on v[0..r).
+// ---------------------------------------------------------------------------
+template <typename R> __host__ __device__ __forceinline__ c2<R> hmk(R a, R b) { c2<R> r; r.x = a; r.y = b; return r; }
+template <typename R> __host__ __device__ __forceinline__ c2<R> hadd(c2<R> a, c2<R> b) { return hmk<R>(a.x + b.x, a.y + b.y); }
+template <typename R> __host__ __device__ __forceinline__ c2<R> hsub(c2<R> a, c2<R> b) { return hmk<R>(a.x - b.x, a.y - b.y); }
+template <typename R> __host__ __device__ __forceinline__ c2<R> hmul(c2<R> a, c2<R> b) {
+  return hmk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename R> __host__ __device__ __forceinline__ c2<R> hmulc(c2<R> a, c2<R> b) {
+  return hmk<R>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// a * exp(-i*theta) (forward) or a * exp(+i*theta) (inverse), given cos, sin of theta
+template <typename R, bool INV>
+__host__ __device__ __forceinline__ c2<R> rot(c2<R> a, R c, R s) {
+  return INV ? hmk<R>(a.x * c - a.y * s, a.y * c + a.x * s) : hmk<R>(a.x * c + a.y * s, a.y * c - a.x * s);
+}
+template <typename R, bool INV> __host__ __device__ __forceinline__ c2<R> mmi(c2<R> a) {
+  return INV ? hmk<R>(-a.y, a.x) : hmk<R>(a.y, -a.x);
+}
+
+template <typename R, bool INV>
+__host__ __device__ __forceinline__ void d2(c2<R>& a, c2<R>& b) {
+  const c2<R> t = a;
+  a = hadd<R>(t, b);
+  b = hsub<R>(t, b);
+}
+template <typename R, bool INV>
+__host__ __device__ __forceinline__ void d3(c2<R>& a0, c2<R>& a1, c2<R>& a2) {
+  const R s3 = R(0.86602540378443864676);
+  const c2<R> t1 = hadd<R>(a1, a2);
+  const c2<R> t2 = hmk<R>(a0.x - R(0.5) * t1.x, a0.y - R(0.5) * t1.y);
+  const c2<R> d = mmi<R, INV>(hsub<R>(a1, a2));
+  const c2<R> t3 = hmk<R>(s3 * d.x, s3 * d.y);
+  a0 = hadd<R>(a0, t1);
+  a1 = hadd<R>(t2, t3);
+  a2 = hsub<R>(t2, t3);
+}
+template <typename R, bool INV>
+__host__ __device__ __forceinline__ void d4(c2<R>& a0, c2<R>& a1, c2<R>& a2, c2<R>& a3) {
+  const c2<R> t0 = hadd<R>(a0, a2), t1 = hsub<R>(a0, a2);
+  const c2<R> t2 = hadd<R>(a1, a3), t3 = mmi<R, INV>(hsub<R>(a1, a3));
+  a0 = hadd<R>(t0, t2);
+  a2 = hsub<R>(t0, t2);
+  a1 = hadd<R>(t1, t3);
+  a3 = hsub<R>(t1, t3);
+}
+template <typename R, bool INV>
+__host__ __device__ __forceinline__ void d5(c2<R>& a0, c2<R>& a1, c2<R>& a2, c2<R>& a3, c2<R>& a4) {
+  const R c1 = R(0.30901699437494742410), c2v = R(-0.80901699437494742410);
+  const R s1 = R(0.95105651629515357212), s2 = R(0.58778525229247312917);
+  const c2<R> b1 = hadd<R>(a1, a4), e1 = hsub<R>(a1, a4);
+  const c2<R> b2 = hadd<R>(a2, a3), e2 = hsub<R>(a2, a3);
+  const c2<R> p1 = hmk<R>(a0.x + c1 * b1.x + c2v * b2.x, a0.y + c1 * b1.y + c2v * b2.y);
+  const c2<R> p2 = hmk<R>(a0.x + c2v * b1.x + c1 * b2.x, a0.y + c2v * b1.y + c1 * b2.y);
+  const c2<R> q1 = mmi<R, INV>(hmk<R>(s1 * e1.x + s2 * e2.x, s1 * e1.y + s2 * e2.y));
+  const c2<R> q2 = mmi<R, INV>(hmk<R>(s2 * e1.x - s1 * e2.x, s2 * e1.y - s1 * e2.y));
+  a0 = hmk<R>(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
+  a1 = hadd<R>(p1, q1);
+  a4 = hsub<R>(p1, q1);
+  a2 = hadd<R>(p2, q2);
+  a3 = hsub<R>(p2, q2);
+}
+
+template <typename R, int r, bool INV> __host__ __device__ __forceinline__ void dftr(c2<R>* v) {
+  if constexpr (r == 2) {
+    d2<R, INV>(v[0], v[1]);
+  } else if constexpr (r == 3) {
+    d3<R, INV>(v[0], v[1], v[2]);
+  } else if constexpr (r == 4) {
+    d4<R, INV>(v[0], v[1], v[2], v[3]);
+  } else if constexpr (r == 5) {
+    d5<R, INV>(v[0], v[1], v[2], v[3], v[4]);
+  } else if constexpr (r == 6) {
+    // 6 = 2 x 3: X[k1 + 2 k2] = sum_n2 W6^{n2 k1} W3^{n2 k2} sum_n1 x[3 n1 + n2] W2^{n1 k1}
+    const R h = R(0.5), s3 = R(0.86602540378443864676);
+    c2<R> a0 = v[0], b0 = v[3], a1 = v[1], b1 = v[4], a2 = v[2], b2 = v[5];
+    d2<R, INV>(a0, b0);
+    d2<R, INV>(a1, b1);
+    d2<R, INV>(a2, b2);
+    b1 = rot<R, INV>(b1, h, s3);
+    b2 = rot<R, INV>(b2, -h, s3);
+    d3<R, INV>(a0, a1, a2);
+    d3<R, INV>(b0, b1, b2);
+    v[0] = a0; v[2] = a1; v[4] = a2;
+    v[1] = b0; v[3] = b1; v[5] = b2;
+  } else if constexpr (r == 8) {
+    const R h = R(0.70710678118654752440);
+    c2<R> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    c2<R> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    d4<R, INV>(e0, e1, e2, e3);
+    d4<R, INV>(o0, o1, o2, o3);
+    o1 = rot<R, INV>(o1, h, h);
+    o2 = mmi<R, INV>(o2);
+    o3 = rot<R, INV>(o3, -h, h);
+    v[0] = hadd<R>(e0, o0); v[4] = hsub<R>(e0, o0);
+    v[1] = hadd<R>(e1, o1); v[5] = hsub<R>(e1, o1);
+    v[2] = hadd<R>(e2, o2); v[6] = hsub<R>(e2, o2);
+    v[3] = hadd<R>(e3, o3); v[7] = hsub<R>(e3, o3);
+  } else if constexpr (r == 10) {
+    // 10 = 2 x 5
+    const R c36 = R(0.80901699437494742410), s36 = R(0.58778525229247312917);
+    const R c72 = R(0.30901699437494742410), s72 = R(0.95105651629515357212);
+    c2<R> a[5], b[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      a[n] = v[n];
+      b[n] = v[n + 5];
+      d2<R, INV>(a[n], b[n]);
+    }
+    b[1] = rot<R, INV>(b[1], c36, s36);
+    b[2] = rot<R, INV>(b[2], c72, s72);
+    b[3] = rot<R, INV>(b[3], -c72, s72);
+    b[4] = rot<R, INV>(b[4], -c36, s36);
+    d5<R, INV>(a[0], a[1], a[2], a[3], a[4]);
+    d5<R, INV>(b[0], b[1], b[2], b[3], b[4]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      v[2 * k] = a[k];
+      v[2 * k + 1] = b[k];
+    }
+  } else if constexpr (r == 12) {
+    // 12 = 4 x 3: X[k1 + 4 k2] = sum_n2 W12^{n2 k1} W3^{n2 k2} sum_n1 x[3 n1 + n2] W4^{n1 k1}
+    const R h = R(0.5), s3 = R(0.86602540378443864676);
+    c2<R> a[3][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+      a[n2][0] = v[n2]; a[n2][1] = v[n2 + 3]; a[n2][2] = v[n2 + 6]; a[n2][3] = v[n2 + 9];
+      d4<R, INV>(a[n2][0], a[n2][1], a[n2][2], a[n2][3]);
+    }
+    a[1][1] = rot<R, INV>(a[1][1], s3, h);
+    a[1][2] = rot<R, INV>(a[1][2], h, s3);
+    a[1][3] = mmi<R, INV>(a[1][3]);
+    a[2][1] = rot<R, INV>(a[2][1], h, s3);
+    a[2][2] = rot<R, INV>(a[2][2], -h, s3);
+    a[2][3] = hmk<R>(-a[2][3].x, -a[2][3].y);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      d3<R, INV>(a[0][k1], a[1][k1], a[2][k1]);
+      v[k1] = a[0][k1]; v[k1 + 4] = a[1][k1]; v[k1 + 8] = a[2][k1];
+    }
+  } else if constexpr (r == 16) {
+    // 16 = 4 x 4
+    const R c1 = R(0.92387953251128675613), s1 = R(0.38268343236508977173);
+    const R h = R(0.70710678118654752440);
+    c2<R> a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      a[n2][0] = v[n2]; a[n2][1] = v[n2 + 4]; a[n2][2] = v[n2 + 8]; a[n2][3] = v[n2 + 12];
+      d4<R, INV>(a[n2][0], a[n2][1], a[n2][2], a[n2][3]);
+    }
+    a[1][1] = rot<R, INV>(a[1][1], c1, s1);
+    a[1][2] = rot<R, INV>(a[1][2], h, h);
+    a[1][3] = rot<R, INV>(a[1][3], s1, c1);
+    a[2][1] = rot<R, INV>(a[2][1], h, h);
+    a[2][2] = mmi<R, INV>(a[2][2]);
+    a[2][3] = rot<R, INV>(a[2][3], -h, h);
+    a[3][1] = rot<R, INV>(a[3][1], s1, c1);
+    a[3][2] = rot<R, INV>(a[3][2], -h, h);
+    a[3][3] = rot<R, INV>(a[3][3], -c1, -s1);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      d4<R, INV>(a[0][k1], a[1][k1], a[2][k1], a[3][k1]);
+      v[k1] = a[0][k1]; v[k1 + 4] = a[1][k1]; v[k1 + 8] = a[2][k1]; v[k1 + 12] = a[3][k1];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The register-resident transform.  v[q*r_p + m] <-> butterfly j = t + q*T,
+// input m of the current pass (on entry, p = 0) / output m of the last pass (on
+// exit).  lds: this line's exchange buffer, element e at lds[e * ES].
+// Every per-pass quantity is a compile-time constant of PassC<N, P>.
+// ---------------------------------------------------------------------------
+template <int N, int P> struct PassC {
+  static constexpr int r = plan_rad(N, P);
+  static constexpr int NB = N / r;
+  static constexpr int LS = plan_ls(N, P);
+  static constexpr int Q = plan_Q(N, P);
+  static constexpr int T = plan_T(N);
+  static constexpr bool FULL = (NB % T) == 0;
+  static __host__ __device__ __forceinline__ int in_idx(int t, int q, int m) { return t + q * T + m * NB; }
+  static __host__ __device__ __forceinline__ int out_idx(int t, int q, int m) {
+    const int j = t + q * T, k = j % LS;
+    return (j - k) * r + k + m * LS;
+  }
+  static __host__ __device__ __forceinline__ bool valid(int t, int q) { return FULL || t + q * T < NB; }
+};
+template <int N> struct RPlan {
+  static constexpr int NP = plan_np(N);
+  static constexpr int T = plan_T(N);
+  static constexpr int E = plan_E(N);
+  static_assert(plan_ok(N), "unsupported transform length");
+};
+
+// One pass's arithmetic on registers: twiddles (p > 0) then radix-r DFTs.
+template <typename R, int N, bool INV, int P>
+__host__ __device__ __forceinline__ void rpass_compute(c2<R>* v, int t, bool act, const c2<R>* __restrict__ tw, int twm) {
+  using C = PassC<N, P>;
+  constexpr int r = C::r, NB = C::NB, LS = C::LS, Q = C::Q, T = C::T;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = t + q * T;
+    if (!act || !((NB % T) == 0 || j < NB)) continue;
+    if constexpr (LS > 1) {
+      const int k = j % LS;
+#pragma unroll
+      for (int m = 1; m < r; ++m) {
+        const c2<R> w = tw[(m * k * (N / (LS * r))) * twm];
+        v[q * r + m] = INV ? hmulc<R>(v[q * r + m], w) : hmul<R>(v[q * r + m], w);
+      }
+    }
+    dftr<R, r, INV>(v + q * r);
+  }
+}
+
+// LDS element stride: compile-time ES, or ES == 0 for a runtime stride es.
+template <int ES> __device__ __forceinline__ int lds_stride(int es) { return ES ? ES : es; }
+
+template <typename R, int N, int P, int ES>
+__device__ __forceinline__ void rpass_store(const c2<R>* v, int t, bool act, c2<R>* lds, int es) {
+  using C = PassC<N, P>;
+  const int st = lds_stride<ES>(es);
+#pragma unroll
+  for (int q = 0; q < C::Q; ++q) {
+    if (!act || !C::valid(t, q)) continue;
+#pragma unroll
+    for (int m = 0; m < C::r; ++m) lds[C::out_idx(t, q, m) * st] = v[q * C::r + m];
+  }
+}
+template <typename R, int N, int P, int ES>
+__device__ __forceinline__ void rpass_load(c2<R>* v, int t, bool act, const c2<R>* lds, int es) {
+  using C = PassC<N, P>;
+  const int st = lds_stride<ES>(es);
+#pragma unroll
+  for (int q = 0; q < C::Q; ++q) {
+    if (!act || !C::valid(t, q)) continue;
+#pragma unroll
+    for (int m = 0; m < C::r; ++m) v[q * C::r + m] = lds[C::in_idx(t, q, m) * st];
+  }
+}
+
+// Passes P.. of the transform; each exchange is bracketed by block barriers.
+template <typename R, int N, bool INV, int ES, int P>
+__device__ __forceinline__ void rfft_from(c2<R>* v, int t, bool act, c2<R>* lds, const c2<R>* __restrict__ tw,
+                                          int twm, int es) {
+  rpass_compute<R, N, INV, P>(v, t, act, tw, twm);
+  if constexpr (P + 1 < plan_np(N)) {
+    __syncthreads();
+    rpass_store<R, N, P, ES>(v, t, act, lds, es);
+    __syncthreads();
+    rpass_load<R, N, P + 1, ES>(v, t, act, lds, es);
+    rfft_from<R, N, INV, ES, P + 1>(v, t, act, lds, tw, twm, es);
+  }
+}
+// ES = 0: the LDS element stride is the runtime value es.
+template <typename R, int N, bool INV, int ES>
+__device__ __forceinline__ void rfft(c2<R>* v, int t, bool act, c2<R>* lds, const c2<R>* __restrict__ tw,
+                                     int twm = 1, int es = 0) {
+  rfft_from<R, N, INV, ES, 0>(v, t, act, lds, tw, twm, es);
+}
+
+// Element index of register slot s after the transform (last pass outputs) and
+// before it (pass-0 inputs); s is a compile-time constant in unrolled loops.
+template <int N> using LastPass = PassC<N, plan_np(N) - 1>;
+template <int N> __host__ __device__ __forceinline__ int rfft_out_idx(int t, int s) {
+  return LastPass<N>::out_idx(t, s / LastPass<N>::r, s % LastPass<N>::r);
+}
+template <int N> __host__ __device__ __forceinline__ int rfft_in_idx(int t, int s) {
+  return PassC<N, 0>::in_idx(t, s / PassC<N, 0>::r, s % PassC<N, 0>::r);
+}
+template <int N> __host__ __device__ __forceinline__ bool rfft_in_valid(int t, int s) {
+  return PassC<N, 0>::valid(t, s / PassC<N, 0>::r);
+}
+template <int N> __host__ __device__ __forceinline__ bool rfft_out_valid(int t, int s) {
+  return LastPass<N>::valid(t, s / LastPass<N>::r);
+}
+// Number of register slots used at the input / output (variable templates, so
+// that loops over them have compile-time trip counts and fully unroll).
+template <int N> constexpr int kInSlots = PassC<N, 0>::Q * PassC<N, 0>::r;
+template <int N> constexpr int kOutSlots = LastPass<N>::Q * LastPass<N>::r;
+// True when the thread's output element set equals its input element set (so a
+// forward transform's outputs can feed an inverse transform without exchange).
+template <int N> __host__ __device__ constexpr bool rfft_io_same() {
+  return plan_np(N) == 1 || (plan_np(N) == 2 && plan_rad(N, 0) == plan_rad(N, 1));
+}
+
+}  // namespace pf

@@ -1,0 +1,442 @@
+// Fused kernels of the reduced timestep.
+//
+// Between steps the work bank W holds every field "x-half-line" state: the full
+// spectrum transformed back along z and y but not x.  One step is four kernels:
+//   k_xline  : per (y,z) line, all fields: C2R along x -> nonlinear terms ->
+//              R2C along x (f_trans normalised), diagnostics of the old state
+//   k_ypass  : forward y (main block columns + Nyquist-plane rows)
+//   k_zline  : per z column tile: forward z -> semi-implicit update against the
+//              carried spectrum S (written back) -> inverse z
+//   k_ypass  : inverse y
+// so each field's half spectrum crosses HBM 10 times per step (x: 2, y: 2+2,
+// z: 4 including S), against 17 for separate transform and pointwise passes.
+#pragma once
+#include "pf_kernels.hpp"
+#include "pf_step_kernels.hpp"
+
+namespace pf {
+
+struct UPar {
+  double dt, kphi, kc, D, L, Dgb, am, ap, cm, cp, s1g, s2g, invP;
+};
+
+// Deterministic block reductions for any blockDim (multiple of 64).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+template <bool MAX> __device__ __forceinline__ double wave_ext(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_down(v, o, 64);
+    if (MAX ? (w > v) : (w < v)) v = w;
+  }
+  return v;
+}
+// red: >= 16 doubles of LDS.  Result valid in thread 0.
+__device__ __forceinline__ double bsum(double v, double* red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+template <bool MAX> __device__ __forceinline__ double bext(double v, double* red) {
+  v = wave_ext<MAX>(v);
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = v;
+  if (threadIdx.x == 0) {
+    s = red[0];
+    for (int i = 1; i < nw; ++i)
+      if (MAX ? (red[i] > s) : (red[i] < s)) s = red[i];
+  }
+  return s;
+}
+
+// Map physical block ids to logical ones so that each XCD (blocks are dealt to
+// the 8 XCDs round-robin) works on a contiguous range: neighbouring lines then
+// share that XCD's L2 for the Nyquist-plane entries.
+__device__ __forceinline__ long xcd_block(long b, long nb) {
+  if (nb % 8) return b;
+  return (b % 8) * (nb / 8) + b / 8;
+}
+
+// Nonlinear terms for one point, reduced O(var) form (gamma block structure of
+// src/var_diff.f90:206-218).  v: field 0 = c, fields 1..V = eta, at stride fs.
+template <typename R, bool DIAG, bool WRITE>
+__device__ __forceinline__ void nl_point(R* v, long fs, int V, const UPar& u, R& mob_out,
+                                         double& a_ch, double& a_eta, double& cmax, double& cmin,
+                                         double& pmin, double& p3) {
+  const R am = (R)u.am, ap = (R)u.ap, cm = (R)u.cm, cp = (R)u.cp, s1g = (R)u.s1g, s2g = (R)u.s2g;
+  const int mat = V - 1;
+  const R con = v[0];
+  R sm = 0, num = 0, pair = 0, fe = 0;
+#pragma unroll 2
+  for (int i = 1; i <= mat; ++i) {
+    const R fi = v[i * fs];
+    const R a = fi * fi;
+    pair += a * sm;
+    sm += a;
+    num += a * a;
+    if (DIAG) fe += a * (R(0.25) * a - R(0.5));
+  }
+  const R fv = v[V * fs], av = fv * fv, av2 = av * av;
+  const R d4 = num + av2;
+  const R hm = num / d4, hp = R(1) - hm;
+  const R dm = con - cm, dp = con - cp;
+  const R fm = am * (dm * dm), fpp = ap * (dp * dp);
+  const R inv_d4 = R(1) / d4;
+  if (WRITE) {
+    v[0] = R(2) * hm * am * dm + R(2) * hp * ap * dp;
+    const R cT4m = R(4) * (fm - fpp) * (av2 * inv_d4) * inv_d4;
+#pragma unroll 2
+    for (int i = 1; i <= mat; ++i) {
+      const R fi = v[i * fs], a = fi * fi;
+      const R ex = s1g * (sm - a) + s2g * av;
+      v[i * fs] = fi * (a - R(1)) + R(2) * fi * ex + cT4m * fi * a;
+    }
+    const R T4v = R(4) * fv * av * hm * (fpp - fm) * inv_d4;
+    v[V * fs] = fv * (av - R(1)) + R(2) * fv * s2g * sm + T4v;
+  }
+  const R ps = sm + av;
+  mob_out = R(2) * (R(1) - ps) * (R)u.Dgb;
+  if (DIAG) {
+    fe += av * (R(0.25) * av - R(0.5)) + s1g * pair + s2g * av * sm;
+    a_ch += (double)(hm * fm + hp * fpp);
+    a_eta += (double)fe;
+    const double dfv = fv, dc = con, dps = ps;
+    if (dfv > p3) p3 = dfv;
+    if (dc <= cmin) cmin = dc;
+    if (dc > cmax) cmax = dc;
+    if (dps < pmin) pmin = dps;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_xline: LINES consecutive (y,z) lines of all F fields per iteration.
+//   wm, wn : W bank (main block, Nyquist plane); in: x-half lines; out: R2C of
+//            the nonlinear derivatives (df_dc, df_dphi_i), divided by P.
+//   mobd   : (m - D_mean) per point when mob (input of the mobility chain).
+//   XONLY  : mobility-chain variant for one field: C2R, multiply by mobd, R2C.
+// One job = one (field, line) x transform, done by T = plan_T(M) threads.  The
+// LDS tile is element-major with the jobs as columns (element e of job j at
+// sh[e*JP + j], JP = jobs rounded up to odd), so lanes of a wave, which run
+// consecutive jobs, touch consecutive LDS words in every phase.
+// Dynamic LDS: (M*JP + M) complex + JP reals + 16 doubles.
+// ---------------------------------------------------------------------------
+template <typename R> constexpr int kXlineMaxThreads = 512;
+// Global loads each thread keeps in flight when staging a line group.
+constexpr int kXlineBatch = 16;
+
+__host__ __device__ constexpr int xline_pitch(int njobs) { return njobs | 1; }
+
+template <typename R, int NX, int LINES, bool XONLY>
+__global__ void __launch_bounds__(kXlineMaxThreads<R>)
+k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __restrict__ mobd,
+        const c2<R>* __restrict__ tw, double* __restrict__ part, long ngroups, int diag, int mob) {
+  constexpr int M = NX / 2, T = plan_T(M), E = plan_E(M);
+  extern __shared__ unsigned char smem[];
+  const int F = XONLY ? 1 : g.F;
+  const int njobs = F * LINES;
+  const int JP = xline_pitch(njobs);
+  c2<R>* sh = reinterpret_cast<c2<R>*>(smem);
+  c2<R>* shw = sh + (long)M * JP;  // W^k, k < M
+  R* shn = reinterpret_cast<R*>(shw + M);
+  double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)M * JP + M) +
+                                                    sizeof(R) * JP) + 15) & ~(size_t)15));
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int job = tid % JP, t = tid / JP;
+  for (int k = tid; k < M; k += nt) shw[k] = tw[k];
+  const bool act = job < njobs && t < T;
+  c2<R>* col = sh + (act ? job : 0);
+  const R scale = (R)u.invP;
+  double a_ch = 0.0, a_eta = 0.0, cmax = 0.0, cmin = 1.0, pmin = 1.0, p3 = 0.0;
+  c2<R> v[E];
+
+  const long lb = xcd_block(blockIdx.x, gridDim.x);
+  const long chunk = (ngroups + gridDim.x - 1) / gridDim.x;
+  const long lg_end = ((lb + 1) * chunk < ngroups) ? (lb + 1) * chunk : ngroups;
+  for (long lg = lb * chunk; lg < lg_end; ++lg) {
+    const long L0 = lg * LINES;
+    // 1. half lines of every field -> LDS (coalesced along k, transposed into the
+    //    tile), kXlineBatch loads in flight per thread
+    for (int i0 = tid; i0 < njobs * M; i0 += kXlineBatch * nt) {
+      c2<R> tmp[kXlineBatch];
+#pragma unroll
+      for (int q = 0; q < kXlineBatch; ++q) {
+        const int i = i0 + q * nt;
+        if (i < njobs * M) {
+          const int jb = i / M, k = i - jb * M;
+          const int f = jb / LINES, l = jb - f * LINES;
+          tmp[q] = wm[(long)f * g.hmain + (L0 + l) * M + k];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kXlineBatch; ++q) {
+        const int i = i0 + q * nt;
+        if (i < njobs * M) {
+          const int jb = i / M, k = i - jb * M;
+          sh[k * JP + jb] = tmp[q];
+        }
+      }
+    }
+    for (int i = tid; i < njobs; i += nt) {
+      const int f = i / LINES, l = i - f * LINES;
+      shn[i] = wn[(long)f * g.plane + L0 + l].x;
+    }
+    __syncthreads();
+    // 2. C2R pre-processing straight into the transform's input registers
+#pragma unroll
+    for (int s = 0; s < kInSlots<M>; ++s) {
+      if (!(act && rfft_in_valid<M>(t, s))) continue;
+      const int k = rfft_in_idx<M>(t, s);
+      if (k == 0) {
+        const R a0 = col[0].x, xn = shn[job];
+        v[s] = mk<R>(a0 + xn, a0 - xn);
+      } else {
+        v[s] = c2r_pre<R>(col[k * JP], col[(M - k) * JP], shw[k]);
+      }
+    }
+    // 3. inverse x transform; real pairs z[n] = (x[2n], x[2n+1]) back into the tile
+    rfft<R, M, true, 0>(v, t, act, col, tw, 2, JP);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kOutSlots<M>; ++s)
+      if (act && rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s) * JP] = v[s];
+    __syncthreads();
+    // 4. pointwise: point x of line l is component x%2 of complex row x/2, column
+    //    f*LINES + l
+    {
+      R* rl = reinterpret_cast<R*>(sh);
+      const long fs = 2L * LINES;  // field stride in reals within a row
+      for (int p = tid; p < LINES * NX; p += nt) {
+        const int l = p / NX, x = p - l * NX;
+        R* vv = rl + 2L * ((long)(x >> 1) * JP + l) + (x & 1);
+        const long gp = (L0 + l) * NX + x;  // point index within a field
+        if (XONLY) {
+          vv[0] = vv[0] * mobd[gp];
+        } else {
+          R mo;
+          if (diag) nl_point<R, true, true>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+          else nl_point<R, false, true>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+          if (mob) mobd[gp] = mo;
+        }
+      }
+    }
+    __syncthreads();
+    // 5. forward x transform
+#pragma unroll
+    for (int s = 0; s < kInSlots<M>; ++s)
+      if (act && rfft_in_valid<M>(t, s)) v[s] = col[rfft_in_idx<M>(t, s) * JP];
+    rfft<R, M, false, 0>(v, t, act, col, tw, 2, JP);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kOutSlots<M>; ++s)
+      if (act && rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s) * JP] = v[s];
+    __syncthreads();
+    // 6. R2C post-processing and store (coalesced along k)
+    const R hs = R(0.5) * scale;
+    for (int i = tid; i < njobs * M; i += nt) {
+      const int jb = i / M, k = i - jb * M;
+      const int f = jb / LINES, l = jb - f * LINES;
+      const int km = (M - k) % M;
+      wm[(long)f * g.hmain + (L0 + l) * M + k] = r2c_post<R>(sh[k * JP + jb], sh[km * JP + jb], shw[k], hs);
+    }
+    for (int i = tid; i < njobs; i += nt) {
+      const int f = i / LINES, l = i - f * LINES;
+      const c2<R> z0 = sh[i];
+      wn[(long)f * g.plane + L0 + l] = mk<R>((z0.x - z0.y) * scale, R(0));
+    }
+    __syncthreads();
+  }
+  if (!XONLY && diag) {
+    const double s0 = bsum(a_ch, red), s1 = bsum(a_eta, red);
+    const double e0 = bext<true>(cmax, red), e1 = bext<false>(cmin, red);
+    const double e2 = bext<false>(pmin, red), e3 = bext<true>(p3, red);
+    if (tid == 0) {
+      double* o = part + (long)blockIdx.x * 6;
+      o[0] = s0; o[1] = s1; o[2] = e0; o[3] = e1; o[4] = e2; o[5] = e3;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_ypass: y transform of the main block (column tiles of W = M, panels (z, f))
+// and of the Nyquist planes (contiguous lines of length NY), one launch.
+// ---------------------------------------------------------------------------
+template <typename R, int NY, bool INV>
+__global__ void __launch_bounds__(col_tile(NY, sizeof(c2<R>)) * plan_T(NY))
+k_ypass(c2<R>* __restrict__ main, c2<R>* __restrict__ nyq, long M, long npanels, long nlines,
+        const c2<R>* __restrict__ tw) {
+  constexpr int T = plan_T(NY), E = plan_E(NY);
+  constexpr int TX = col_tile(NY, sizeof(c2<R>));
+  __shared__ c2<R> sh[(plan_np(NY) > 1) ? NY * TX : 1];
+  const long tpp = (M + TX - 1) / TX;
+  const long nmain = tpp * npanels;
+  const long b = blockIdx.x;
+  c2<R> v[E];
+  if (b < nmain) {
+    const long pnl = b / tpp;
+    const long col0 = (b - pnl * tpp) * TX;
+    const int ncol = (int)((M - col0) < TX ? (M - col0) : TX);
+    const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+    const bool act = c < ncol;
+    c2<R>* gp = main + pnl * NY * M + col0 + c;
+#pragma unroll
+    for (int s = 0; s < kInSlots<NY>; ++s)
+      if (act && rfft_in_valid<NY>(t, s)) v[s] = gp[(long)rfft_in_idx<NY>(t, s) * M];
+    rfft<R, NY, INV, TX>(v, t, act, sh + c, tw);
+#pragma unroll
+    for (int s = 0; s < kOutSlots<NY>; ++s)
+      if (act && rfft_out_valid<NY>(t, s)) gp[(long)rfft_out_idx<NY>(t, s) * M] = v[s];
+  } else {
+    const int l = threadIdx.x / T, t = threadIdx.x % T;
+    const long line = (b - nmain) * TX + l;
+    const bool act = line < nlines;
+    c2<R>* gp = nyq + (act ? line : 0) * NY;
+#pragma unroll
+    for (int s = 0; s < kInSlots<NY>; ++s)
+      if (act && rfft_in_valid<NY>(t, s)) v[s] = gp[rfft_in_idx<NY>(t, s)];
+    rfft<R, NY, INV, 1>(v, t, act, sh + ((plan_np(NY) > 1) ? l * NY : 0), tw);
+#pragma unroll
+    for (int s = 0; s < kOutSlots<NY>; ++s)
+      if (act && rfft_out_valid<NY>(t, s)) gp[rfft_out_idx<NY>(t, s)] = v[s];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_zline: forward z -> update -> inverse z on column tiles of each field.
+// ROLE_REG : c (D_gb = 0 form) and eta updates; W -> S, W
+// ROLE_GHAT: c only; W_c -> f^ (stored to X) and g^ = kf_sum f^ - kappa_c c^ k_cube,
+//            inverse z of g^ -> Y
+// ROLE_CMOB: c only; h^ from Y (forward z) with f^ from X -> S_c, W_c
+// The spectrum values a thread holds after the forward transform are updated in
+// registers (S read and written at the same rows); when the transform's output
+// rows differ from its input rows an LDS exchange precedes the inverse.
+// ---------------------------------------------------------------------------
+enum { ROLE_REG = 0, ROLE_GHAT = 1, ROLE_CMOB = 2 };
+
+template <typename R> __device__ __forceinline__ R recip(R x) { return R(1) / x; }
+template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <typename R, int NZ, int ROLE, bool DC64, bool PARSEVAL>
+__global__ void __launch_bounds__(col_tile(NZ, sizeof(c2<R>)) * plan_T(NZ))
+k_zline(Geo g, UPar u, typename KTabOf<R>::T kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
+        double* __restrict__ dc64, double* __restrict__ part, const c2<R>* __restrict__ tw, int f0, int nf,
+        long tm, long tn) {
+  constexpr int TX = col_tile(NZ, sizeof(c2<R>));
+  constexpr int NI = kInSlots<NZ>, NO = kOutSlots<NZ>, E = plan_E(NZ);
+  __shared__ c2<R> sh[(plan_np(NZ) > 1) ? NZ * TX : 1];
+  __shared__ double red[16];
+  const long tpf = tm + tn;
+  const long total = (long)nf * tpf;
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const R Dk = (R)((u.D * u.dt) * u.kc), Ddt = (R)(u.D * u.dt), Lkd = (R)((u.L * u.kphi) * u.dt);
+  const R Ldt = (R)(u.L * u.dt), dtR = (R)u.dt, kcR = (R)u.kc;
+  double acc = 0.0;
+  c2<R> v[E];
+  c2<R> sv[NO];
+  c2<R> xv[(ROLE == ROLE_CMOB) ? NO : 1];
+  for (long tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    const int f = f0 + (int)(tile / tpf);
+    const long tt = tile % tpf;
+    const bool isn = tt >= tm;
+    const long Wd = isn ? (long)g.ny : (long)g.M * g.ny;
+    const long col0 = (isn ? tt - tm : tt) * TX;
+    const int ncol = (int)((Wd - col0) < TX ? (Wd - col0) : TX);
+    const bool act = c < ncol;
+    const long off = (isn ? (long)f * g.plane : (long)f * g.hmain) + col0 + c;
+    const long off0 = col0 + c;  // single-field banks X, Y (c-only roles)
+    c2<R>* const wb = (isn ? W.nyq : W.main) + off;
+    c2<R>* const sb = (isn ? S.nyq : S.main) + off;
+    c2<R>* const xb = (isn ? X.nyq : X.main) + off0;
+    const c2<R>* const ib = (ROLE == ROLE_CMOB) ? (isn ? Y.nyq : Y.main) + off0 : wb;
+    // 1. all loads of the tile at once: transform input, spectrum (and f^) at the output rows
+#pragma unroll
+    for (int s = 0; s < NI; ++s)
+      if (act && rfft_in_valid<NZ>(t, s)) v[s] = ib[(long)rfft_in_idx<NZ>(t, s) * Wd];
+#pragma unroll
+    for (int s = 0; s < NO; ++s)
+      if (act && rfft_out_valid<NZ>(t, s)) {
+        sv[s] = sb[(long)rfft_out_idx<NZ>(t, s) * Wd];
+        if constexpr (ROLE == ROLE_CMOB) xv[s] = xb[(long)rfft_out_idx<NZ>(t, s) * Wd];
+      }
+    const long col = col0 + c;
+    const int kx = isn ? g.M : (int)(col % g.M);
+    const int ky = isn ? (int)col : (int)(col / g.M);
+    const KCol<R> kc = kcol<R>(kt, kx, ky);
+    const R kap = (f == 0) ? (R)u.kc : (R)u.kphi;
+    // 2. forward z, update in registers
+    rfft<R, NZ, false, TX>(v, t, act, sh + c, tw);
+#pragma unroll
+    for (int s = 0; s < NO; ++s) {
+      if (!(act && rfft_out_valid<NZ>(t, s))) continue;
+      const int e = rfft_out_idx<NZ>(t, s);
+      const KCoefR<R> k = kcoef_col<R>(kt, kc, e);
+      const c2<R> w = v[s];
+      const long ga = (long)e * Wd;
+      c2<R> sn;
+      if constexpr (ROLE == ROLE_GHAT) {
+        const c2<R> ch = sv[s];
+        xb[ga] = w;  // f^
+        sn = mk<R>(-(k.s1 * w.y) + kcR * ch.y * k.k3, k.s1 * w.x - kcR * ch.x * k.k3);
+      } else {
+        const c2<R> s0 = sv[s];
+        if (f == 0) {
+          const R A = recip<R>(R(1) + Dk * k.k4), B = (Ddt * k.k2) * A;
+          if constexpr (ROLE == ROLE_CMOB) {
+            const c2<R> fh = xv[s];
+            const R Cc = (dtR * k.s1) * A;
+            sn = mk<R>(A * s0.x + B * fh.x - Cc * w.y, A * s0.y + B * fh.y + Cc * w.x);
+          } else {
+            sn = mk<R>(A * s0.x + B * w.x, A * s0.y + B * w.y);
+          }
+        } else {
+          const R A = recip<R>(R(1) - Lkd * k.k2), B = -Ldt * A;
+          sn = mk<R>(A * s0.x + B * w.x, A * s0.y + B * w.y);
+        }
+        if (DC64 && kx == 0 && ky == 0 && e == 0) {
+          if (f != 0) dc64[f] = dc64[f] - (u.L * u.dt) * (double)w.x;
+          sn = mk<R>((R)dc64[f], R(0));
+        }
+        sb[ga] = sn;
+        if (PARSEVAL) {
+          const R wgt = (kx == 0 || kx == g.M) ? R(1) : R(2);
+          acc += (double)((R(0.5) * kap) * wgt * k.kp2 * (sn.x * sn.x + sn.y * sn.y));
+        }
+      }
+      v[s] = sn;
+    }
+    // 3. inverse z (exchange first when the output rows are not the input rows)
+    if constexpr (!rfft_io_same<NZ>()) {
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < NO; ++s)
+        if (act && rfft_out_valid<NZ>(t, s)) sh[rfft_out_idx<NZ>(t, s) * TX + c] = v[s];
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < NI; ++s)
+        if (act && rfft_in_valid<NZ>(t, s)) v[s] = sh[rfft_in_idx<NZ>(t, s) * TX + c];
+    }
+    rfft<R, NZ, true, TX>(v, t, act, sh + c, tw);
+    c2<R>* const ob = (ROLE == ROLE_GHAT) ? (isn ? Y.nyq : Y.main) + off0 : wb;
+#pragma unroll
+    for (int s = 0; s < NO; ++s)
+      if (act && rfft_out_valid<NZ>(t, s)) ob[(long)rfft_out_idx<NZ>(t, s) * Wd] = v[s];
+    __syncthreads();
+  }
+  if (PARSEVAL) {
+    const double s = bsum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s * (double)g.P;
+  }
+}
+
+}  // namespace pf

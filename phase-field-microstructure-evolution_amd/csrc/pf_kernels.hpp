@@ -11,22 +11,111 @@
 // Batches of fields are contiguous: field f's main block starts at f*M*NY*NZ and
 // its Nyquist plane at f*NY*NZ, so "lines" of the x pass are numbered
 // L = y + NY*(z + NZ*f) across the batch.
+//
+// Transforms use the register-resident Stockham of pf_fft2.hpp: pass-0 inputs
+// are loaded from HBM straight into registers and last-pass outputs are stored
+// from registers; LDS is only used for the transposes between passes (and, in
+// the x pass, for the real<->half-complex pairing of k and M-k).
 #pragma once
-#include "pf_fft.hpp"
+#include "pf_fft2.hpp"
 
 namespace pf {
 
 constexpr int kBlock = 256;
 
-__host__ __device__ constexpr int lines_per_block(int N) {
-  return (kBlock / fft_threads(N)) > 0 ? (kBlock / fft_threads(N)) : 1;
-}
-// columns per tile of a column pass: >= 16 for 128-byte runs, LDS tile <= 64 KiB
+// columns per tile of a column pass: 16..64 columns, block of ~256 threads,
+// exchange tile <= 64 KiB
 __host__ __device__ constexpr int col_tile(int N, int cbytes) {
-  int tx = kBlock / fft_threads(N);
+  int tx = kBlock / plan_T(N);
   if (tx < 16) tx = 16;
+  if (tx > 64) tx = 64;
   while (tx > 1 && N * tx * cbytes > 65536) tx /= 2;
   return tx;
+}
+// lines per block of a contiguous-line pass
+__host__ __device__ constexpr int lines_per_block(int N) {
+  return (kBlock / plan_T(N)) > 0 ? (kBlock / plan_T(N)) : 1;
+}
+
+// ---------------------------------------------------------------------------
+// Column pass: FFT along the rows of row-major panels (N rows x W columns),
+// batch of panels at stride N*W.  Used for y (W = M, one panel per (z, field))
+// and z (W = M*NY, one panel per field; W = NY for the Nyquist planes).
+// ---------------------------------------------------------------------------
+template <typename R, int N, bool INV>
+__global__ void __launch_bounds__(col_tile(N, sizeof(c2<R>)) * plan_T(N))
+k_col_fft(c2<R>* __restrict__ data, long W, long tiles_per_panel, const c2<R>* __restrict__ tw) {
+  constexpr int T = plan_T(N), E = plan_E(N);
+  constexpr int TX = col_tile(N, sizeof(c2<R>));
+  __shared__ c2<R> sh[(plan_np(N) > 1) ? N * TX : 1];
+  const long b = blockIdx.x / tiles_per_panel;
+  const long col0 = (blockIdx.x - b * tiles_per_panel) * TX;
+  const int ncol = (int)((W - col0) < TX ? (W - col0) : TX);
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const bool act = c < ncol;
+  c2<R>* g = data + b * N * W + col0 + c;
+  c2<R> v[E];
+#pragma unroll
+  for (int s = 0; s < kInSlots<N>; ++s)
+    if (act && rfft_in_valid<N>(t, s)) v[s] = g[(long)rfft_in_idx<N>(t, s) * W];
+  rfft<R, N, INV, TX>(v, t, act, sh + c, tw);
+#pragma unroll
+  for (int s = 0; s < kOutSlots<N>; ++s)
+    if (act && rfft_out_valid<N>(t, s)) g[(long)rfft_out_idx<N>(t, s) * W] = v[s];
+}
+
+// ---------------------------------------------------------------------------
+// Row pass: FFT of contiguous complex lines of length N (Nyquist-plane y pass).
+// ---------------------------------------------------------------------------
+template <typename R, int N, bool INV>
+__global__ void __launch_bounds__(lines_per_block(N) * plan_T(N))
+k_row_fft(c2<R>* __restrict__ data, long nlines, const c2<R>* __restrict__ tw) {
+  constexpr int T = plan_T(N), E = plan_E(N);
+  constexpr int LPB = lines_per_block(N);
+  __shared__ c2<R> sh[(plan_np(N) > 1) ? LPB * N : 1];
+  const int l = threadIdx.x / T, t = threadIdx.x % T;
+  const long line = (long)blockIdx.x * LPB + l;
+  const bool act = line < nlines;
+  c2<R>* g = data + (act ? line : 0) * N;
+  c2<R> v[E];
+#pragma unroll
+  for (int s = 0; s < kInSlots<N>; ++s)
+    if (act && rfft_in_valid<N>(t, s)) v[s] = g[rfft_in_idx<N>(t, s)];
+  rfft<R, N, INV, 1>(v, t, act, sh + ((plan_np(N) > 1) ? l * N : 0), tw);
+#pragma unroll
+  for (int s = 0; s < kOutSlots<N>; ++s)
+    if (act && rfft_out_valid<N>(t, s)) g[rfft_out_idx<N>(t, s)] = v[s];
+}
+
+// ---------------------------------------------------------------------------
+// Real <-> half-complex x lines on a complex FFT of length M = NX/2.
+// z[n] = x[2n] + i x[2n+1];  Z = FFT_M(z);  W = exp(-2 pi i/NX).
+//   R2C:  X_k = 1/2 (Z_k + conj Z_{M-k}) - i/2 W^k (Z_k - conj Z_{M-k}),  k = 0..M
+//   C2R:  Z_k = (X_k + conj X_{M-k}) + i conj(W^k) (X_k - conj X_{M-k}), k = 0..M-1,
+//         with the imaginary parts of X_0 and X_M dropped (Hermitian projection).
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ c2<R> r2c_post(c2<R> zk, c2<R> zmk, c2<R> wk, R half_scale) {
+  const R ar = zk.x + zmk.x, ai = zk.y - zmk.y;
+  const R br = zk.x - zmk.x, bi = zk.y + zmk.y;
+  const R cr = wk.x * br - wk.y * bi, ci = wk.x * bi + wk.y * br;
+  return mk<R>(half_scale * (ar + ci), half_scale * (ai - cr));
+}
+template <typename R>
+__device__ __forceinline__ c2<R> c2r_pre(c2<R> xk, c2<R> xmk, c2<R> wk) {
+  const R ar = xk.x + xmk.x, ai = xk.y - xmk.y;
+  const R br = xk.x - xmk.x, bi = xk.y + xmk.y;
+  const R cr = wk.x * br + wk.y * bi, ci = wk.x * bi - wk.y * br;
+  return mk<R>(ar - ci, ai + cr);
+}
+// Z_k for C2R from the half line x (M entries) and the Nyquist real xn.
+template <typename R, int M>
+__device__ __forceinline__ c2<R> c2r_z(const c2<R>* x, R xn, int k, const c2<R>* __restrict__ tw) {
+  if (k == 0) {
+    const R a0 = x[0].x;
+    return mk<R>(a0 + xn, a0 - xn);
+  }
+  return c2r_pre<R>(x[k], x[M - k], tw[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -35,21 +124,28 @@ __host__ __device__ constexpr int col_tile(int N, int cbytes) {
 // tw:   table of exp(-2 pi i q/NX), q in [0, NX).
 // ---------------------------------------------------------------------------
 template <typename R, int NX>
-__global__ void __launch_bounds__(lines_per_block(NX / 2) * fft_threads(NX / 2))
+__global__ void __launch_bounds__(lines_per_block(NX / 2) * plan_T(NX / 2))
 k_x_r2c(c2<R>* __restrict__ data, c2<R>* __restrict__ nyq, long nlines,
         const c2<R>* __restrict__ tw, R scale) {
-  constexpr int M = NX / 2;
-  constexpr int T = fft_threads(M);
-  constexpr int LPB = lines_per_block(M);
-  constexpr int NT = LPB * T;
+  constexpr int M = NX / 2, T = plan_T(M), E = plan_E(M);
+  constexpr int LPB = lines_per_block(M), NT = LPB * T;
   __shared__ c2<R> sh[LPB * M];
   const long l0 = (long)blockIdx.x * LPB;
   const int nl = (int)((nlines - l0) < LPB ? (nlines - l0) : LPB);
-  c2<R>* g = data + l0 * M;
-  for (int i = threadIdx.x; i < nl * M; i += NT) sh[i] = g[i];
-  __syncthreads();
   const int l = threadIdx.x / T, t = threadIdx.x % T;
-  lds_fft<R, M, false, 1, 2>(sh + l * M, t, l < nl, tw);
+  const bool act = l < nl;
+  c2<R>* g = data + l0 * M;
+  c2<R> v[E];
+#pragma unroll
+  for (int s = 0; s < kInSlots<M>; ++s)
+    if (act && rfft_in_valid<M>(t, s)) v[s] = g[(long)l * M + rfft_in_idx<M>(t, s)];
+  c2<R>* line = sh + l * M;
+  rfft<R, M, false, 1>(v, t, act, line, tw, 2);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kOutSlots<M>; ++s)
+    if (act && rfft_out_valid<M>(t, s)) line[rfft_out_idx<M>(t, s)] = v[s];
+  __syncthreads();
   const R hs = R(0.5) * scale;
   for (int i = threadIdx.x; i < nl * M; i += NT) {
     const int ll = i / M, k = i - ll * M;
@@ -66,90 +162,33 @@ k_x_r2c(c2<R>* __restrict__ data, c2<R>* __restrict__ nyq, long nlines,
 // x pass, inverse: half spectrum lines -> real lines (unnormalised).
 // ---------------------------------------------------------------------------
 template <typename R, int NX>
-__global__ void __launch_bounds__(lines_per_block(NX / 2) * fft_threads(NX / 2))
+__global__ void __launch_bounds__(lines_per_block(NX / 2) * plan_T(NX / 2))
 k_x_c2r(c2<R>* __restrict__ data, const c2<R>* __restrict__ nyq, long nlines,
         const c2<R>* __restrict__ tw) {
-  constexpr int M = NX / 2;
-  constexpr int T = fft_threads(M);
+  constexpr int M = NX / 2, T = plan_T(M), E = plan_E(M);
   constexpr int LPB = lines_per_block(M);
-  constexpr int NT = LPB * T;
-  __shared__ c2<R> sh[LPB * M];
-  __shared__ R shn[LPB];
-  const long l0 = (long)blockIdx.x * LPB;
-  const int nl = (int)((nlines - l0) < LPB ? (nlines - l0) : LPB);
-  c2<R>* g = data + l0 * M;
-  for (int i = threadIdx.x; i < nl * M; i += NT) sh[i] = g[i];
-  if (threadIdx.x < nl) shn[threadIdx.x] = nyq[l0 + threadIdx.x].x;
-  __syncthreads();
-  // pre-processing: each thread owns a pair (k, M-k)
-  constexpr int NP = M / 2 + 1;
-  for (int i = threadIdx.x; i < nl * NP; i += NT) {
-    const int ll = i / NP, p = i - ll * NP;
-    c2<R>* x = sh + ll * M;
-    if (p == 0) {
-      const R a0 = x[0].x, am = shn[ll];
-      x[0] = mk<R>(a0 + am, a0 - am);
+  __shared__ c2<R> sh[(plan_np(M) > 1) ? LPB * M : 1];
+  const int l = threadIdx.x / T, t = threadIdx.x % T;
+  const long line = (long)blockIdx.x * LPB + l;
+  const bool act = line < nlines;
+  c2<R>* g = data + (act ? line : 0) * M;
+  c2<R> v[E];
+  // pre-processing from HBM straight into registers (X_k and X_{M-k} of the line)
+#pragma unroll
+  for (int s = 0; s < kInSlots<M>; ++s) {
+    if (!(act && rfft_in_valid<M>(t, s))) continue;
+    const int k = rfft_in_idx<M>(t, s);
+    if (k == 0) {
+      const R a0 = g[0].x, xn = nyq[line].x;
+      v[s] = mk<R>(a0 + xn, a0 - xn);
     } else {
-      const int k1 = p, k2 = M - p;
-      const c2<R> x1 = x[k1], x2 = x[k2];
-      x[k1] = c2r_pre<R>(x1, x2, tw[k1]);
-      if (k2 != k1) x[k2] = c2r_pre<R>(x2, x1, tw[k2]);
+      v[s] = c2r_pre<R>(g[k], g[M - k], tw[k]);
     }
   }
-  __syncthreads();
-  const int l = threadIdx.x / T, t = threadIdx.x % T;
-  lds_fft<R, M, true, 1, 2>(sh + l * M, t, l < nl, tw);
-  for (int i = threadIdx.x; i < nl * M; i += NT) g[i] = sh[i];
-}
-
-// ---------------------------------------------------------------------------
-// Column pass: FFT along the rows of row-major panels (N rows x W columns),
-// batch of panels at stride N*W.  Used for y (W = M, one panel per (z, field))
-// and z (W = M*NY, one panel per field; W = NY for the Nyquist planes).
-// ---------------------------------------------------------------------------
-template <typename R, int N, bool INV>
-__global__ void __launch_bounds__(col_tile(N, sizeof(c2<R>)) * fft_threads(N))
-k_col_fft(c2<R>* __restrict__ data, long W, long tiles_per_panel,
-          const c2<R>* __restrict__ tw) {
-  constexpr int T = fft_threads(N);
-  constexpr int TX = col_tile(N, sizeof(c2<R>));
-  constexpr int NT = TX * T;
-  __shared__ c2<R> sh[N * TX];
-  const long b = blockIdx.x / tiles_per_panel;
-  const long col0 = (blockIdx.x - b * tiles_per_panel) * TX;
-  const int ncol = (int)((W - col0) < TX ? (W - col0) : TX);
-  c2<R>* g = data + b * N * W + col0;
-  for (int i = threadIdx.x; i < N * TX; i += NT) {
-    const int e = i / TX, c = i % TX;
-    if (c < ncol) sh[i] = g[e * W + c];
-  }
-  __syncthreads();
-  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  lds_fft<R, N, INV, TX, 1>(sh + c, t, c < ncol, tw);
-  for (int i = threadIdx.x; i < N * TX; i += NT) {
-    const int e = i / TX, cc = i % TX;
-    if (cc < ncol) g[e * W + cc] = sh[i];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Row pass: FFT of contiguous complex lines of length N (Nyquist-plane y pass).
-// ---------------------------------------------------------------------------
-template <typename R, int N, bool INV>
-__global__ void __launch_bounds__(lines_per_block(N) * fft_threads(N))
-k_row_fft(c2<R>* __restrict__ data, long nlines, const c2<R>* __restrict__ tw) {
-  constexpr int T = fft_threads(N);
-  constexpr int LPB = lines_per_block(N);
-  constexpr int NT = LPB * T;
-  __shared__ c2<R> sh[LPB * N];
-  const long l0 = (long)blockIdx.x * LPB;
-  const int nl = (int)((nlines - l0) < LPB ? (nlines - l0) : LPB);
-  c2<R>* g = data + l0 * N;
-  for (int i = threadIdx.x; i < nl * N; i += NT) sh[i] = g[i];
-  __syncthreads();
-  const int l = threadIdx.x / T, t = threadIdx.x % T;
-  lds_fft<R, N, INV, 1, 1>(sh + l * N, t, l < nl, tw);
-  for (int i = threadIdx.x; i < nl * N; i += NT) g[i] = sh[i];
+  rfft<R, M, true, 1>(v, t, act, sh + ((plan_np(M) > 1) ? l * M : 0), tw, 2);
+#pragma unroll
+  for (int s = 0; s < kOutSlots<M>; ++s)
+    if (act && rfft_out_valid<M>(t, s)) g[rfft_out_idx<M>(t, s)] = v[s];
 }
 
 }  // namespace pf

@@ -11,6 +11,7 @@
 
 #include "../../include/pf.h"
 #include "pf_internal.hpp"
+#include "pf_fused.hpp"
 #include "pf_step_kernels.hpp"
 
 namespace pf {
@@ -28,9 +29,7 @@ constexpr int NB = 256;        // block size of pointwise kernels
 constexpr int NPART_NL = 6;    // f_ch, f_eta, c_max, c_min, phi_min, phi_3
 constexpr long kMaxGrid = 8192;
 
-struct UCoef {
-  double dt, kphi, kc, D, L, Dgb, am, ap, cm, cp, s1g, s2g, invP;
-};
+using UCoef = UPar;
 
 // ===========================================================================
 // Kernels
@@ -428,7 +427,9 @@ struct pf_handle {
   long step = 0;  // completed steps
 
   double* dk = nullptr;
+  float* dkf = nullptr;
   KTab kt{};
+  KTabF ktf{};
   double* dgam = nullptr;
 
   // faithful (FP64) banks
@@ -437,6 +438,7 @@ struct pf_handle {
   // reduced banks (precision R, stored untyped)
   void *S_main = nullptr, *S_nyq = nullptr, *W_main = nullptr, *W_nyq = nullptr;
   void *G_main = nullptr, *G_nyq = nullptr, *mob = nullptr;
+  void *Fc_main = nullptr, *Fc_nyq = nullptr, *X_main = nullptr, *X_nyq = nullptr;
   double* dc64 = nullptr;
 
   double* part_nl = nullptr;
@@ -460,6 +462,10 @@ struct pf_handle {
 };
 
 namespace {
+
+template <typename R> typename KTabOf<R>::T ktab(const pf_handle* h);
+template <> KTab ktab<double>(const pf_handle* h) { return h->kt; }
+template <> KTabF ktab<float>(const pf_handle* h) { return h->ktf; }
 
 int dalloc(pf_handle* h, void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
@@ -553,6 +559,7 @@ int step_faithful(pf_handle* h, bool diag, long rec) {
   int rc;
   double* st = (double*)h->fs.main;
   // loops 5-10 (read the real state before it is transformed in place)
+  h->n_part_nl = grid_for(g.P, NB);
   tbeg(h, ST_NONLINEAR);
   const size_t shb = sizeof(double) * ((size_t)g.var * NB + NB);
   if (diag)
@@ -612,8 +619,120 @@ int step_faithful(pf_handle* h, bool diag, long rec) {
 }
 
 // ---------------------------------------------------------------------------
-// Reduced step: W holds the real state on entry and on exit; S the spectrum.
+// Reduced step (fused kernels).  Between steps W holds the x-half-line state
+// (the spectrum S transformed back along z and y), S the full spectrum.
 // ---------------------------------------------------------------------------
+#define PF_GRID_SIZES(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(160) X(192) X(256) X(512)
+
+// Lines per x-line block: small lines are grouped so that a block has work for
+// ~128 points; a second instantiation with twice as many lines is used when it
+// fits (threads, 64 KiB of LDS).
+constexpr int xline_lines(int NX) {
+  int l = 1;
+  while (l * 2 * NX <= 256) l *= 2;
+  return l;
+}
+
+template <typename R, int NX, int LINES, bool XONLY>
+bool xline_fits(const Geo& g, long* nt_out, size_t* shb_out) {
+  constexpr int M = NX / 2, T = plan_T(M);
+  const int F = XONLY ? 1 : g.F;
+  if (g.plane % LINES) return false;
+  const long jobs = (long)F * LINES;
+  const long JP = xline_pitch((int)jobs);
+  long need = std::max<long>(64, (JP * T + 63) / 64 * 64);
+  const size_t body = sizeof(c2<R>) * ((size_t)M * JP + M) + sizeof(R) * (size_t)JP;
+  const size_t shb = ((body + 15) & ~(size_t)15) + 16 * sizeof(double);
+  if (need > kXlineMaxThreads<R> || shb > 65536) return false;
+  *nt_out = need;
+  *shb_out = shb;
+  return true;
+}
+
+template <typename R, int NX, int LINES, bool XONLY>
+int launch_xline_n(pf_handle* h, c2<R>* wm, c2<R>* wn, R* mobd, double* part, long* grid_out, bool diag,
+                   bool mob, long nt, size_t shb) {
+  const Geo& g = h->g;
+  const long ngroups = g.plane / LINES;
+  long grid = std::min<long>(ngroups, kMaxGrid);
+  if (grid >= 8) grid -= grid % 8;
+  const c2<R>* tw = twiddle_table<R>(NX);
+  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  hipLaunchKernelGGL((k_xline<R, NX, LINES, XONLY>), dim3((unsigned)grid), dim3((unsigned)nt), shb, h->s, g, h->u,
+                     wm, wn, mobd, tw, part, ngroups, (int)diag, (int)mob);
+  if (grid_out) *grid_out = grid;
+  return 0;
+}
+
+template <typename R, bool XONLY>
+int launch_xline(pf_handle* h, c2<R>* wm, c2<R>* wn, R* mobd, double* part, long* grid_out, bool diag,
+                 bool mob) {
+  long nt = 0;
+  size_t shb = 0;
+  switch (h->g.nx) {
+#define PF_CASE(N)                                                                                        \
+  case N:                                                                                                 \
+    if (xline_fits<R, N, 2 * xline_lines(N), XONLY>(h->g, &nt, &shb))                                     \
+      return launch_xline_n<R, N, 2 * xline_lines(N), XONLY>(h, wm, wn, mobd, part, grid_out, diag, mob, nt, shb); \
+    if (xline_fits<R, N, xline_lines(N), XONLY>(h->g, &nt, &shb))                                         \
+      return launch_xline_n<R, N, xline_lines(N), XONLY>(h, wm, wn, mobd, part, grid_out, diag, mob, nt, shb); \
+    if (xline_fits<R, N, 1, XONLY>(h->g, &nt, &shb))                                                      \
+      return launch_xline_n<R, N, 1, XONLY>(h, wm, wn, mobd, part, grid_out, diag, mob, nt, shb);        \
+    return fail(PF_ENOTIMPL, "x-line kernel: " + std::to_string(XONLY ? 1 : h->g.F) +                     \
+                                 " fields of length " #N " exceed one block's threads or 64 KiB of LDS");
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported nx");
+  }
+}
+
+template <typename R, bool INV>
+int launch_ypass(pf_handle* h, c2<R>* main, c2<R>* nyq, int nf) {
+  const Geo& g = h->g;
+  switch (g.ny) {
+#define PF_CASE(N)                                                                              \
+  case N: {                                                                                     \
+    constexpr int TX = col_tile(N, sizeof(c2<R>));                                              \
+    const long npanels = (long)g.nz * nf, nlines = (long)g.nz * nf;                             \
+    const long tpp = ((long)g.M + TX - 1) / TX;                                                 \
+    const long grid = tpp * npanels + (nlines + TX - 1) / TX;                                   \
+    const c2<R>* tw = twiddle_table<R>(N);                                                      \
+    if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");                           \
+    hipLaunchKernelGGL((k_ypass<R, N, INV>), dim3((unsigned)grid), dim3(TX * plan_T(N)), 0, \
+                       h->s, main, nyq, (long)g.M, npanels, nlines, tw);                        \
+    return 0;                                                                                   \
+  }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported ny");
+  }
+}
+
+template <typename R, int ROLE, bool DC64, bool PARSEVAL>
+int launch_zline(pf_handle* h, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y, double* part, int f0, int nf,
+                 long* grid_out) {
+  const Geo& g = h->g;
+  switch (g.nz) {
+#define PF_CASE(N)                                                                                   \
+  case N: {                                                                                          \
+    constexpr int TX = col_tile(N, sizeof(c2<R>));                                                   \
+    const long tm = ((long)g.M * g.ny + TX - 1) / TX, tn = ((long)g.ny + TX - 1) / TX;               \
+    const long grid = std::min<long>((tm + tn) * nf, kMaxGrid);                                      \
+    const c2<R>* tw = twiddle_table<R>(N);                                                           \
+    if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");                                \
+    hipLaunchKernelGGL((k_zline<R, N, ROLE, DC64, PARSEVAL>), dim3((unsigned)grid),                  \
+                       dim3(TX * plan_T(N)), 0, h->s, g, h->u, ktab<R>(h), S, W, X, Y, h->dc64, part, tw, \
+                       f0, nf, tm, tn);                                                              \
+    if (grid_out) *grid_out = grid;                                                                  \
+    return 0;                                                                                        \
+  }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported nz");
+  }
+}
+
 template <typename R>
 int step_reduced(pf_handle* h, bool diag, long rec, bool fint_next) {
   const Geo& g = h->g;
@@ -621,9 +740,7 @@ int step_reduced(pf_handle* h, bool diag, long rec, bool fint_next) {
   const bool mob = h->p.D_gb != 0.0;
   constexpr bool DC = sizeof(R) == 4;
   Bank<R> S{(c2<R>*)h->S_main, (c2<R>*)h->S_nyq}, W{(c2<R>*)h->W_main, (c2<R>*)h->W_nyq};
-  Bank<R> G{(c2<R>*)h->G_main, (c2<R>*)h->G_nyq};
-  const R invP = (R)(1.0 / (double)(float)g.P);
-  const unsigned rb = row_block(g);
+  Bank<R> G{(c2<R>*)h->G_main, (c2<R>*)h->G_nyq}, Fc{(c2<R>*)h->Fc_main, (c2<R>*)h->Fc_nyq};
   int rc;
 
   if (diag && !h->fint_ready) {
@@ -634,70 +751,51 @@ int step_reduced(pf_handle* h, bool diag, long rec, bool fint_next) {
     h->n_part_fi = grid_for(rows, 1);
     tend(h);
   }
+  // x: C2R -> nonlinear -> R2C, all fields
   tbeg(h, ST_NONLINEAR);
   {
-    const size_t shb = sizeof(R) * (size_t)g.var * NB + 16 + sizeof(double) * NB;
-    R* wr = (R*)h->W_main;
+    long gr = 0;
     R* md = (R*)h->mob;
-    const dim3 gr(grid_for(g.P, NB));
-#define PF_NL(D, M) hipLaunchKernelGGL((k_nl_reduced<R, D, M>), gr, dim3(NB), shb, s, g, h->u, wr, md, h->part_nl)
-    if (diag) { if (mob) PF_NL(true, true); else PF_NL(true, false); }
-    else { if (mob) PF_NL(false, true); else PF_NL(false, false); }
-#undef PF_NL
+    rc = launch_xline<R, false>(h, W.main, W.nyq, md, h->part_nl, &gr, diag, mob);
+    if (rc) return rc;
+    if (diag) h->n_part_nl = gr;
   }
   tend(h);
   if (diag) {
     tbeg(h, ST_DIAG);
-    if (DC) diag_final(h, rec, true, true, h->dc64, nullptr);
-    else diag_final(h, rec, true, true, (const double*)h->S_main, nullptr);
+    diag_final(h, rec, true, true, DC ? h->dc64 : (const double*)h->S_main, nullptr);
     tend(h);
   }
-  // forward transforms of df_dc, df_dphi (with f_trans normalisation)
-  const long M = g.M;
-  tbeg(h, ST_X_R2C);
-  if ((rc = x_r2c<R>(g.nx, W.main, W.nyq, (long)g.ny * g.nz * g.F, invP, s))) return rc;
-  tend(h);
   tbeg(h, ST_Y_FWD);
-  if ((rc = col_pass<R>(g.ny, false, W.main, M, (long)g.nz * g.F, s))) return rc;
-  if ((rc = row_pass<R>(g.ny, false, W.nyq, (long)g.nz * g.F, s))) return rc;
+  if ((rc = launch_ypass<R, false>(h, W.main, W.nyq, g.F))) return rc;
   tend(h);
-  tbeg(h, ST_Z_FWD);
-  if ((rc = col_pass<R>(g.nz, false, W.main, M * g.ny, g.F, s))) return rc;
-  if ((rc = col_pass<R>(g.nz, false, W.nyq, g.ny, g.F, s))) return rc;
-  tend(h);
-  if (mob) {
-    // variable-mobility correction (loops 11-12) for c
-    tbeg(h, ST_MOBILITY);
-    hipLaunchKernelGGL((k_ghat<R>), dim3(grid_for(g.plane, 1)), dim3(rb), 0, s, g, h->u, h->kt, W, 0, S, 0, G,
-                       (long)g.plane);
-    if ((rc = inv3d<R>(G.main, G.nyq, g.nx, g.ny, g.nz, 1, s))) return rc;
-    hipLaunchKernelGGL((k_mult<R>), dim3(grid_for(g.P, NB)), dim3(NB), 0, s, g.P, (R*)G.main, (const R*)h->mob);
-    if ((rc = fwd3d<R>(G.main, G.nyq, g.nx, g.ny, g.nz, 1, invP, s))) return rc;
-    tend(h);
-  }
   tbeg(h, ST_UPDATE);
-  {
-    const long rows = (long)g.plane * g.F;
-    const dim3 gr(grid_for(rows, 1));
-#define PF_UP(MB, PV) \
-  hipLaunchKernelGGL((k_update<R, DC, MB, PV, true>), gr, dim3(NB), 0, s, g, h->u, h->kt, S, W, G, h->dc64, h->part_fi, rows)
-    if (mob) { if (fint_next) PF_UP(true, true); else PF_UP(true, false); }
-    else { if (fint_next) PF_UP(false, true); else PF_UP(false, false); }
-#undef PF_UP
-    if (fint_next) h->n_part_fi = gr.x;
-    h->fint_ready = fint_next;
+  long g1 = 0, g2 = 0;
+  if (!mob) {
+    rc = fint_next ? launch_zline<R, ROLE_REG, DC, true>(h, S, W, W, W, h->part_fi, 0, g.F, &g1)
+                   : launch_zline<R, ROLE_REG, DC, false>(h, S, W, W, W, h->part_fi, 0, g.F, &g1);
+    if (rc) return rc;
+  } else {
+    if ((rc = launch_zline<R, ROLE_GHAT, false, false>(h, S, W, Fc, G, h->part_fi, 0, 1, nullptr))) return rc;
+    rc = fint_next ? launch_zline<R, ROLE_REG, DC, true>(h, S, W, W, W, h->part_fi, 1, g.var, &g1)
+                   : launch_zline<R, ROLE_REG, DC, false>(h, S, W, W, W, h->part_fi, 1, g.var, &g1);
+    if (rc) return rc;
+    tend(h);
+    tbeg(h, ST_MOBILITY);
+    if ((rc = launch_ypass<R, true>(h, G.main, G.nyq, 1))) return rc;
+    if ((rc = launch_xline<R, true>(h, G.main, G.nyq, (R*)h->mob, nullptr, nullptr, false, false))) return rc;
+    if ((rc = launch_ypass<R, false>(h, G.main, G.nyq, 1))) return rc;
+    tend(h);
+    tbeg(h, ST_UPDATE);
+    rc = fint_next ? launch_zline<R, ROLE_CMOB, DC, true>(h, S, W, Fc, G, h->part_fi + g1, 0, 1, &g2)
+                   : launch_zline<R, ROLE_CMOB, DC, false>(h, S, W, Fc, G, h->part_fi + g1, 0, 1, &g2);
+    if (rc) return rc;
   }
-  tend(h);
-  tbeg(h, ST_Z_INV);
-  if ((rc = col_pass<R>(g.nz, true, W.main, M * g.ny, g.F, s))) return rc;
-  if ((rc = col_pass<R>(g.nz, true, W.nyq, g.ny, g.F, s))) return rc;
+  if (fint_next) h->n_part_fi = g1 + g2;
+  h->fint_ready = fint_next;
   tend(h);
   tbeg(h, ST_Y_INV);
-  if ((rc = col_pass<R>(g.ny, true, W.main, M, (long)g.nz * g.F, s))) return rc;
-  if ((rc = row_pass<R>(g.ny, true, W.nyq, (long)g.nz * g.F, s))) return rc;
-  tend(h);
-  tbeg(h, ST_X_C2R);
-  if ((rc = x_c2r<R>(g.nx, W.main, W.nyq, (long)g.ny * g.nz * g.F, s))) return rc;
+  if ((rc = launch_ypass<R, true>(h, W.main, W.nyq, g.F))) return rc;
   tend(h);
   return 0;
 }
@@ -706,12 +804,16 @@ template <typename R> int alloc_reduced(pf_handle* h) {
   Bank<R> S{}, W{}, G{};
   int rc = alloc_bank<R>(h, S, h->g.F);
   if (!rc) rc = alloc_bank<R>(h, W, h->g.F);
+  Bank<R> Fc{}, X{};
   if (!rc && h->p.D_gb != 0.0) {
     rc = alloc_bank<R>(h, G, 1);
+    if (!rc) rc = alloc_bank<R>(h, Fc, 1);
     if (!rc) rc = dalloc(h, &h->mob, sizeof(R) * (size_t)h->g.P);
   }
+  if (!rc) rc = alloc_bank<R>(h, X, h->g.F);
   h->S_main = S.main; h->S_nyq = S.nyq; h->W_main = W.main; h->W_nyq = W.nyq;
-  h->G_main = G.main; h->G_nyq = G.nyq;
+  h->G_main = G.main; h->G_nyq = G.nyq; h->Fc_main = Fc.main; h->Fc_nyq = Fc.nyq;
+  h->X_main = X.main; h->X_nyq = X.nyq;
   if (!rc) rc = dalloc(h, (void**)&h->dc64, sizeof(double) * h->g.F);
   return rc;
 }
@@ -723,6 +825,9 @@ template <typename R> int put_reduced(pf_handle* h, const double* phi, const dou
                      h->staging + (size_t)g.var * g.P, (R*)h->W_main, g.P, g.var);
   PF_HIP(hipMemcpyAsync(h->S_main, h->W_main, sizeof(R) * (size_t)g.P * g.F, hipMemcpyDeviceToDevice, h->s));
   int rc = fwd3d<R>((c2<R>*)h->S_main, (c2<R>*)h->S_nyq, g.nx, g.ny, g.nz, g.F, (R)(1.0 / (double)(float)g.P), h->s);
+  if (rc) return rc;
+  // W <- x-half lines: (1/nx) R2C along x of the real state == z,y-inverse of S
+  rc = x_r2c<R>(g.nx, (c2<R>*)h->W_main, (c2<R>*)h->W_nyq, (long)g.ny * g.nz * g.F, (R)(1.0 / (double)g.nx), h->s);
   if (rc) return rc;
   // FP64 means (k = 0 modes), Neumaier-compensated
   std::vector<double> dc(g.F, 0.0);
@@ -787,6 +892,14 @@ extern "C" int pf_create(const pf_params* p, int nx, int ny, int nz, int var, in
     h->kt.ax = d; h->kt.axp = d + M + 1; d += 2 * (M + 1);
     h->kt.ay = d; h->kt.ayp = d + ny; d += 2 * ny;
     h->kt.az = d; h->kt.azp = d + nz;
+    std::vector<float> tabf(tab.begin(), tab.end());
+    if ((rc = dalloc(h, (void**)&h->dkf, sizeof(float) * tabf.size()))) return bail(rc);
+    if (hipMemcpy(h->dkf, tabf.data(), sizeof(float) * tabf.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(PF_EHIP, "copy of wavenumber tables failed"));
+    const float* df = h->dkf;
+    h->ktf.ax = df; h->ktf.axp = df + M + 1; df += 2 * (M + 1);
+    h->ktf.ay = df; h->ktf.ayp = df + ny; df += 2 * ny;
+    h->ktf.az = df; h->ktf.azp = df + nz;
   }
   if (mode == PF_MODE_FP64_FAITHFUL) {
     std::vector<double> gam((size_t)var * var);
@@ -804,7 +917,7 @@ extern "C" int pf_create(const pf_params* p, int nx, int ny, int nz, int var, in
     if ((rc = alloc_reduced<float>(h))) return bail(rc);
   }
   if ((rc = dalloc(h, (void**)&h->part_nl, sizeof(double) * NPART_NL * kMaxGrid))) return bail(rc);
-  if ((rc = dalloc(h, (void**)&h->part_fi, sizeof(double) * kMaxGrid))) return bail(rc);
+  if ((rc = dalloc(h, (void**)&h->part_fi, sizeof(double) * 2 * kMaxGrid))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->staging, sizeof(double) * (size_t)g.P * g.F))) return bail(rc);
   h->n_part_nl = grid_for(g.P, NB);
   if ((rc = ensure_diag(h, 64))) return bail(rc);
@@ -839,20 +952,34 @@ extern "C" int pf_put_state(pf_handle* h, const double* phi, const double* con) 
   return put_reduced<float>(h, phi, con);
 }
 
+namespace {
+// Reduced modes: real state of the current step = C2R along x of a copy of W.
+template <typename R> int real_state_reduced(pf_handle* h) {
+  const Geo& g = h->g;
+  PF_HIP(hipMemcpyAsync(h->X_main, h->W_main, sizeof(R) * (size_t)g.P * g.F, hipMemcpyDeviceToDevice, h->s));
+  PF_HIP(hipMemcpyAsync(h->X_nyq, h->W_nyq, sizeof(c2<R>) * (size_t)g.plane * g.F, hipMemcpyDeviceToDevice, h->s));
+  return x_c2r<R>(g.nx, (c2<R>*)h->X_main, (const c2<R>*)h->X_nyq, (long)g.ny * g.nz * g.F, h->s);
+}
+}  // namespace
+
 extern "C" int pf_get_state(pf_handle* h, double* phi, double* con) {
   if (!h || !phi || !con) return fail(PF_EINVAL, "null argument");
   const Geo& g = h->g;
   double* sphi = h->staging;
   double* scon = h->staging + (size_t)g.var * g.P;
-  if (h->mode == PF_MODE_FP64_FAITHFUL)
+  int rc = 0;
+  if (h->mode == PF_MODE_FP64_FAITHFUL) {
     hipLaunchKernelGGL((k_gather_state<double>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s,
                        (const double*)h->fs.main, sphi, scon, g.P, g.var);
-  else if (h->mode == PF_MODE_FP64_REDUCED)
+  } else if (h->mode == PF_MODE_FP64_REDUCED) {
+    if ((rc = real_state_reduced<double>(h))) return rc;
     hipLaunchKernelGGL((k_gather_state<double>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s,
-                       (const double*)h->W_main, sphi, scon, g.P, g.var);
-  else
+                       (const double*)h->X_main, sphi, scon, g.P, g.var);
+  } else {
+    if ((rc = real_state_reduced<float>(h))) return rc;
     hipLaunchKernelGGL((k_gather_state<float>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s,
-                       (const float*)h->W_main, sphi, scon, g.P, g.var);
+                       (const float*)h->X_main, sphi, scon, g.P, g.var);
+  }
   PF_HIP(hipGetLastError());
   PF_HIP(hipMemcpyAsync(phi, sphi, sizeof(double) * (size_t)g.P * g.var, hipMemcpyDeviceToHost, h->s));
   PF_HIP(hipMemcpyAsync(con, scon, sizeof(double) * (size_t)g.P, hipMemcpyDeviceToHost, h->s));
@@ -911,7 +1038,6 @@ extern "C" int pf_diagnose(pf_handle* h, pf_diag* out) {
   hipStream_t s = h->s;
   int rc = ensure_diag(h, 1);
   if (rc) return rc;
-  const unsigned rb = row_block(g);
   const long rows = (long)g.plane * g.F;
   const unsigned gr_nl = grid_for(g.P, NB);
   if (h->mode == PF_MODE_FP64_FAITHFUL) {
@@ -919,6 +1045,7 @@ extern "C" int pf_diagnose(pf_handle* h, pf_diag* out) {
     hipLaunchKernelGGL((k_nl_faithful<true>), dim3(gr_nl), dim3(NB), shb, s, g, h->u, h->dgam,
                        (const double*)h->fs.main, (double*)h->fdfdc.main, (double*)h->fpsi.main, h->fmob,
                        h->part_nl);
+    h->n_part_nl = gr_nl;
     // f_int by Parseval on a transformed copy of the state (same quantity as loops 1-4)
     PF_HIP(hipMemcpyAsync(h->fgrad.main, h->fs.main, sizeof(double) * (size_t)g.P * g.F,
                           hipMemcpyDeviceToDevice, s));
@@ -935,17 +1062,20 @@ extern "C" int pf_diagnose(pf_handle* h, pf_diag* out) {
       hipLaunchKernelGGL((k_update<float, false, false, true, false>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
                          h->u, h->kt, S, S, S, h->dc64, h->part_fi, rows);
       const size_t shb = sizeof(float) * (size_t)g.var * NB + 16 + sizeof(double) * NB;
+      if ((rc = real_state_reduced<float>(h))) return rc;
       hipLaunchKernelGGL((k_nl_reduced<float, true, false, false>), dim3(gr_nl), dim3(NB), shb, s, g, h->u,
-                         (float*)h->W_main, (float*)nullptr, h->part_nl);
+                         (float*)h->X_main, (float*)nullptr, h->part_nl);
     } else {
       Bank<double> S{(c2<double>*)h->S_main, (c2<double>*)h->S_nyq};
       hipLaunchKernelGGL((k_update<double, false, false, true, false>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
                          h->u, h->kt, S, S, S, h->dc64, h->part_fi, rows);
       const size_t shb = sizeof(double) * (size_t)g.var * NB + 16 + sizeof(double) * NB;
+      if ((rc = real_state_reduced<double>(h))) return rc;
       hipLaunchKernelGGL((k_nl_reduced<double, true, false, false>), dim3(gr_nl), dim3(NB), shb, s, g, h->u,
-                         (double*)h->W_main, (double*)nullptr, h->part_nl);
+                         (double*)h->X_main, (double*)nullptr, h->part_nl);
     }
     h->n_part_fi = grid_for(rows, 1);
+    h->n_part_nl = gr_nl;
     h->fint_ready = true;
     diag_final(h, 0, true, true, f32 ? h->dc64 : (const double*)h->S_main, nullptr);
   }

@@ -9,6 +9,42 @@ namespace pf {
 struct KTab {
   const double *ax, *axp, *ay, *ayp, *az, *azp;
 };
+// The same tables rounded to FP32, for the FP32-mixed update arithmetic.
+struct KTabF {
+  const float *ax, *axp, *ay, *ayp, *az, *azp;
+};
+template <typename R> struct KTabOf;
+template <> struct KTabOf<double> { using T = KTab; };
+template <> struct KTabOf<float> { using T = KTabF; };
+
+// Per-column wavenumbers (kx, ky fixed) and the kz-dependent completion, in
+// precision R: s1 = kf_sum/i, k2 = k_sq, k3 = k_cube/i, k4 = k_4, kp2 = |kf|^2.
+template <typename R> struct KCol {
+  R ax, axp, ay, ayp, x2, y2, x4, y4;
+};
+template <typename R, typename KT>
+__device__ __forceinline__ KCol<R> kcol(const KT& t, int kx, int ky) {
+  KCol<R> c;
+  c.ax = t.ax[kx]; c.axp = t.axp[kx]; c.ay = t.ay[ky]; c.ayp = t.ayp[ky];
+  c.x2 = c.ax * c.ax; c.y2 = c.ay * c.ay;
+  c.x4 = (c.x2 * c.ax) * c.ax; c.y4 = (c.y2 * c.ay) * c.ay;
+  return c;
+}
+template <typename R> struct KCoefR {
+  R s1, k2, k3, k4, kp2;
+};
+template <typename R, typename KT>
+__device__ __forceinline__ KCoefR<R> kcoef_col(const KT& t, const KCol<R>& c, int kz) {
+  const R az = t.az[kz], azp = t.azp[kz];
+  const R z2 = az * az;
+  KCoefR<R> k;
+  k.s1 = (c.axp + c.ayp) + azp;
+  k.k2 = (-c.x2 + -c.y2) + -z2;
+  k.k3 = (-(c.axp * c.x2) + -(c.ayp * c.y2)) + -(azp * z2);
+  k.k4 = (c.x4 + c.y4) + (z2 * az) * az;
+  k.kp2 = (c.axp * c.axp + c.ayp * c.ayp) + azp * azp;
+  return k;
+}
 
 // Per-point combinations in the reference's evaluation order
 // (src/var_diff.f90:932-935, 978-981, 1019):

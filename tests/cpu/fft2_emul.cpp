@@ -1,0 +1,95 @@
+// CPU emulation of the register-resident Stockham schedule in pf_fft2.hpp:
+// every thread of a line runs each pass's arithmetic on its registers, and the
+// LDS exchanges between passes are replayed sequentially.  Output: one line per
+// (N, direction) with the max error against a brute-force long-double DFT.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <vector>
+
+#include "../../phase-field-microstructure-evolution_amd/csrc/pf_fft2.hpp"
+
+namespace pf {  // host twiddles, same construction as the library's
+void host_twiddles(int n, double* re, double* im) {
+  const long double two_pi = 6.283185307179586476925286766559L;
+  for (long t = 0; t < n; ++t) {
+    const long q = (4 * t) / n, num = 4 * t - q * n;
+    long double c, s;
+    if (2 * num <= n) { const long double a = two_pi * num / (4.0L * n); c = cosl(a); s = sinl(a); }
+    else { const long double a = two_pi * (n - num) / (4.0L * n); c = sinl(a); s = cosl(a); }
+    const double cr = (double)c, ci = -(double)s;
+    double wr, wi;
+    switch (q) { case 0: wr = cr; wi = ci; break; case 1: wr = ci; wi = -cr; break;
+                 case 2: wr = -cr; wi = -ci; break; default: wr = -ci; wi = cr; }
+    re[t] = wr; im[t] = wi;
+  }
+}
+}  // namespace pf
+
+using namespace pf;
+typedef std::complex<long double> cl;
+
+template <int N, bool INV, int P>
+void passes(std::vector<std::vector<double2>>& regs, const double2* tw) {
+  constexpr int T = plan_T(N);
+  for (int t = 0; t < T; ++t) rpass_compute<double, N, INV, P>(regs[t].data(), t, true, tw, 1);
+  if constexpr (P + 1 < plan_np(N)) {
+    std::vector<double2> lds(N);
+    constexpr int r = plan_rad(N, P), Q = plan_Q(N, P);
+    for (int t = 0; t < T; ++t)
+      for (int q = 0; q < Q; ++q)
+        if (PassC<N, P>::valid(t, q))
+          for (int m = 0; m < r; ++m) lds[PassC<N, P>::out_idx(t, q, m)] = regs[t][q * r + m];
+    constexpr int r1 = plan_rad(N, P + 1), Q1 = plan_Q(N, P + 1);
+    for (int t = 0; t < T; ++t)
+      for (int q = 0; q < Q1; ++q)
+        if (PassC<N, P + 1>::valid(t, q))
+          for (int m = 0; m < r1; ++m) regs[t][q * r1 + m] = lds[PassC<N, P + 1>::in_idx(t, q, m)];
+    passes<N, INV, P + 1>(regs, tw);
+  }
+}
+
+template <int N, bool INV> double check() {
+  std::vector<double> re(N), im(N);
+  host_twiddles(N, re.data(), im.data());
+  std::vector<double2> tw(N);
+  for (int i = 0; i < N; ++i) { tw[i].x = re[i]; tw[i].y = im[i]; }
+  std::vector<double2> x(N);
+  unsigned s = 12345u + N;
+  for (int i = 0; i < N; ++i) {
+    s = s * 1103515245u + 12345u; x[i].x = ((s >> 8) % 20001) / 10000.0 - 1.0;
+    s = s * 1103515245u + 12345u; x[i].y = ((s >> 8) % 20001) / 10000.0 - 1.0;
+  }
+  constexpr int T = plan_T(N), E = plan_E(N);
+  std::vector<std::vector<double2>> regs(T, std::vector<double2>(E));
+  for (int t = 0; t < T; ++t)
+    for (int sl = 0; sl < kInSlots<N>; ++sl)
+      if (rfft_in_valid<N>(t, sl)) regs[t][sl] = x[rfft_in_idx<N>(t, sl)];
+  passes<N, INV, 0>(regs, tw.data());
+  std::vector<double2> y(N);
+  std::vector<int> seen(N, 0);
+  for (int t = 0; t < T; ++t)
+    for (int sl = 0; sl < kOutSlots<N>; ++sl)
+      if (rfft_out_valid<N>(t, sl)) { y[rfft_out_idx<N>(t, sl)] = regs[t][sl]; seen[rfft_out_idx<N>(t, sl)]++; }
+  double err = 0.0;
+  const long double pi2 = 6.283185307179586476925286766559L;
+  for (int k = 0; k < N; ++k) {
+    if (seen[k] != 1) return 1e300;
+    cl acc = 0;
+    for (int n = 0; n < N; ++n) {
+      const long double a = (INV ? 1 : -1) * pi2 * (long double)((long)n * k % N) / N;
+      acc += cl(x[n].x, x[n].y) * cl(cosl(a), sinl(a));
+    }
+    err = std::fmax(err, (double)std::abs(acc - cl(y[k].x, y[k].y)));
+  }
+  return err;
+}
+
+
+#define RUN(N) \
+  std::printf("%d %d %.3e %.3e\n", N, plan_np(N), check<N, false>(), check<N, true>());
+int main() {
+  RUN(2) RUN(3) RUN(4) RUN(5) RUN(6) RUN(8) RUN(10) RUN(12) RUN(16) RUN(20) RUN(24) RUN(32) RUN(40)
+  RUN(48) RUN(64) RUN(80) RUN(96) RUN(128) RUN(160) RUN(192) RUN(256) RUN(512)
+  return 0;
+}

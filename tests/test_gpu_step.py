@@ -55,3 +55,45 @@ def test_fp64_reduced_vs_oracle(pf, oracle, shape, var, nseeds, dgb):
 @pytest.mark.parametrize("dgb", [0.0, 0.3])
 def test_fp32_mixed_vs_oracle(pf, oracle, dgb):
     _compare(pf, oracle, (32, 32, 32), 4, 8, 20, "fp32_mixed", dict(VOR, D_gb=dgb), 1e-4, 1e-4)
+
+
+@pytest.mark.parametrize("mode", ["fp64_faithful", "fp64_reduced", "fp32_mixed"])
+def test_diagnose_matches_next_step_record(pf, mode):
+    """pf_diagnose reports the start-of-step diagnostics of the current state, i.e.
+    the record pf_step would produce for the next step."""
+    n, var = 16, 4
+    phi, con = pf.ic_voronoi(n, n, n, var, 6)
+    params = pf.make_params(t_step=0.005, grad_coeff_phi=1.0, grad_coeff_c=1.0)
+    with pf.Simulation(params, n, n, n, var, mode) as sim:
+        sim.put_state(phi, con)
+        sim.step(3)
+        d0 = sim.diagnose()
+        d1 = sim.step(1, diag_every=1)[0]
+    tol = 1e-12 if mode != "fp32_mixed" else 1e-5
+    for k in ("f_int", "f_ch", "f_eta", "e_tot", "c_max", "c_min", "phi_min", "phi_3_max", "c_mean"):
+        assert abs(d0[k] - d1[k]) <= tol * max(1.0, abs(d1[k])), (k, d0[k], d1[k])
+    assert d0["step"] == d1["step"] == 4
+
+
+@pytest.mark.parametrize("mode", ["fp64_faithful", "fp64_reduced"])
+def test_split_run_equals_continuous_run(pf, mode):
+    """2k steps == k steps, get_state, new handle, put_state, k steps (FP64 forms)."""
+    n, var, k = 16, 3, 6
+    phi, con = pf.ic_voronoi(n, n, n, var, 5)
+    params = pf.make_params(t_step=0.005, grad_coeff_phi=1.0, grad_coeff_c=1.0, D_gb=0.2)
+    with pf.Simulation(params, n, n, n, var, mode) as a:
+        a.put_state(phi, con)
+        a.step(2 * k)
+        pa, ca = a.get_state()
+    with pf.Simulation(params, n, n, n, var, mode) as b:
+        b.put_state(phi, con)
+        b.step(k)
+        pm, cm = b.get_state()
+    with pf.Simulation(params, n, n, n, var, mode) as c:
+        c.put_state(pm, cm)
+        c.step(k)
+        pb, cb = c.get_state()
+    if mode == "fp64_faithful":
+        assert np.array_equal(pa, pb) and np.array_equal(ca, cb)
+    else:  # the carried spectrum is rebuilt from the real state at put_state
+        assert np.abs(pa - pb).max() < 1e-13 and np.abs(ca - cb).max() < 1e-13
