@@ -15,9 +15,38 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "pf_fft.hpp"
 
 namespace pf {
+
+// Host: twiddles exp(-2 pi i t/n), t < n, evaluated in long double on the first
+// octant and mapped by exact quadrant symmetries, then rounded once to double.
+inline void host_twiddles(int n, double* re, double* im) {
+  const long double two_pi = 6.283185307179586476925286766559L;
+  for (long t = 0; t < n; ++t) {
+    const long q = (4 * t) / n;           // quadrant
+    const long num = 4 * t - q * n;       // angle 2 pi num / (4n) in [0, pi/2)
+    long double c, s;
+    if (2 * num <= n) {
+      const long double a = two_pi * (long double)num / (long double)(4L * n);
+      c = cosl(a); s = sinl(a);
+    } else {
+      const long double a = two_pi * (long double)(n - num) / (long double)(4L * n);
+      c = sinl(a); s = cosl(a);
+    }
+    const double cr = (double)c, ci = -(double)s;  // e^{-i a}
+    double wr, wi;
+    switch (q) {  // times (-i)^q
+      case 0: wr = cr; wi = ci; break;
+      case 1: wr = ci; wi = -cr; break;
+      case 2: wr = -cr; wi = -ci; break;
+      default: wr = -ci; wi = cr; break;
+    }
+    re[t] = wr; im[t] = wi;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Plans.

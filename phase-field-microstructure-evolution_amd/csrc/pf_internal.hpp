@@ -55,7 +55,4 @@ template <typename R>
 int unpack_spec(const c2<R>* dense, c2<R>* main, c2<R>* nyq, int nx, int ny, int nz,
                 int batch, int layout, hipStream_t s);
 
-// Host: exp(-2 pi i t/n) with octant reduction, in double.
-void host_twiddles(int n, double* re, double* im);
-
 }  // namespace pf

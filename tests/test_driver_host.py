@@ -1,0 +1,81 @@
+"""CPU tests of the Fortran host driver (var_diff_hip.x): input parsing,
+gamma_matrix.dat, the initial conditions and the restart reader, run with
+PF_DRY_RUN so that no GPU is touched.  The checker is the restatement in
+oracle/ics.py of src/var_diff.f90:229-461 with the oracle's ran_2."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import ics
+from driver_util import DRIVER, cfg_of, run_driver, write_input
+
+pytestmark = pytest.mark.skipif(not os.path.exists(DRIVER), reason="var_diff_hip.x not built")
+
+
+def _dry(tmp_path, **kw):
+    inp = tmp_path / "input.txt"
+    p = write_input(inp, **kw)
+    out = tmp_path / "out"
+    r = run_driver(str(tmp_path), str(inp), str(out), env={"PF_DRY_RUN": "1"})
+    assert r.returncode == 0, r.stderr
+    ist, ien, phi, con = ics.read_checkpoint(str(out / "step_0000000" / "data_save.0001"),
+                                              p["Nx"], p["Ny"], p["Nz"], p["var"])
+    assert list(ist) == [1, 1, 1] and list(ien) == [p["Nx"], p["Ny"], p["Nz"]]
+    return p, phi, con, out, r
+
+
+@pytest.mark.parametrize("shape,num_ppt", [((24, 24, 24), 3), ((32, 16, 24), 2)])
+def test_circle_ic_matches_restatement(tmp_path, shape, num_ppt):
+    p, phi, con, _, r = _dry(tmp_path, Nx=shape[0], Ny=shape[1], Nz=shape[2], num_ppt=num_ppt,
+                             grn_rad=5, start="circle")
+    ephi = np.zeros_like(phi)
+    econ = np.full_like(con, p["c_mat"])
+    ics.circle(cfg_of(p), ephi, econ, oracle.Ran2(p["iseed"]))
+    assert np.array_equal(phi, ephi)
+    assert np.array_equal(con, econ)
+    assert (phi[..., 2] == 1.0).sum() > 0 and r.stdout.count("nppt=") == num_ppt
+
+
+def test_plane_ic_matches_restatement(tmp_path):
+    p, phi, con, _, _ = _dry(tmp_path, Nx=24, Ny=32, Nz=16, start="plane", i_ppt=10, j_ppt=20,
+                             k_ppt=8, ppt_rad=(3, 2, 2))
+    ephi = np.zeros_like(phi)
+    ics.plane(cfg_of(p), ephi)
+    assert np.array_equal(phi, ephi)
+    assert np.all(con == p["c_mat"])
+
+
+def test_restart_reader_roundtrip(tmp_path):
+    import __graft_entry__
+    pf = __graft_entry__.load_package()
+    phi0, con0 = pf.ic_voronoi(16, 24, 12, 4, 5)
+    ics.write_restart(str(tmp_path / "data_rerun.0001"), phi0, con0)
+    p, phi, con, _, _ = _dry(tmp_path, nrun=2, Nx=16, Ny=24, Nz=12, var=4)
+    assert np.array_equal(phi, phi0) and np.array_equal(con, con0)
+
+
+def test_gamma_matrix_file(tmp_path):
+    p, _, _, out, _ = _dry(tmp_path, var=5, sigma_1=1.25, sigma_2=0.75)
+    rows = [l.split() for l in open(out / "gamma_matrix.dat") if l.strip()]
+    got = np.array([[float(x) for x in r[1:]] for r in rows])
+    assert [int(r[0]) for r in rows] == list(range(1, 6))
+    assert np.array_equal(got, ics.gamma(5, 1.25, 0.75))
+
+
+def test_input_comment_and_headers(tmp_path):
+    """The '/' ending each record makes the trailing text a comment; output
+    files carry the reference's header lines."""
+    _, _, _, out, _ = _dry(tmp_path)
+    assert open(out / "summary.dat").readline().startswith("# step number, f_int_tot")
+    assert open(out / "time_step.dat").readline().startswith("# step number")
+    assert open(out / "timing_data.dat").readline().startswith("# step number, loop 1 avg")
+
+
+def test_checkpoint_reader_rejects_truncation(tmp_path):
+    ics.write_fortran_records(str(tmp_path / "f"), [b"\x00" * 24, b"\x01" * 80])
+    data = open(tmp_path / "f", "rb").read()
+    open(tmp_path / "g", "wb").write(data[:-6])
+    with pytest.raises(Exception):
+        ics.read_fortran_records(str(tmp_path / "g"))
