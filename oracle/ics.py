@@ -95,12 +95,19 @@ def circle(cfg, phi, con, rng):
         nppt += 1
 
 
-def insert(cfg, phi, rng):
+def insert(cfg, phi, rng, max_draws=1_000_000):
+    """The reference redraws until a centre passes both tests and has no bound
+    on the number of draws (it never ends when no site qualifies, e.g. once the
+    precipitate field is non-zero everywhere); here max_draws ends it with an error."""
     nx, ny, nz, var = cfg["nx"], cfg["ny"], cfg["nz"], cfg["var"]
     rad = sum(r * r for r in cfg["ppt_rad"])
     nppt = 1
+    draws = 0
     while nppt <= cfg["num_ppt"]:
         while True:
+            draws += 1
+            if draws > max_draws:
+                raise RuntimeError("no admissible precipitate site found")
             ix, jy, kz = _draw(rng, nx), _draw(rng, ny), _draw(rng, nz)
             s = 0.0
             if 1 <= ix <= nx and 1 <= jy <= ny and 1 <= kz <= nz:

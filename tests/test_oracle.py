@@ -20,6 +20,7 @@ import pytest
 
 import oracle
 from oracle import ics
+from closed_forms import ac_harmonics_case, ch_mode_case, single_mode_fint_case
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -173,87 +174,32 @@ def test_gamma_structure():
 # ---------------------------------------------------------------------------
 # whole steps: closed forms
 # ---------------------------------------------------------------------------
-def _axis_a(m, n, zero_nyq):
-    if zero_nyq and 2 * m == n:
-        return 0.0
-    ms = m if m <= n // 2 else m - n
-    return ms * (2 * math.pi / n)
-
-
-def ch_mode_factor(p, mode, shape, mu0):
-    """Per-step multiplier of a single c mode for var = 1 (then h = 0 and
-    df/dc = 2 ap (c - c_p) is linear), uniform phi (so m - D_mean = mu0)."""
-    (mx, my, mz), (nx, ny, nz) = mode, shape
-    a = [_axis_a(mx, nx, False), _axis_a(my, ny, False), _axis_a(mz, nz, False)]
-    ap_ = [_axis_a(mx, nx, True), _axis_a(my, ny, True), _axis_a(mz, nz, True)]
-    s1 = sum(ap_)
-    k2 = -sum(t * t for t in a)
-    k3 = -sum(q * t * t for q, t in zip(ap_, a))
-    k4 = sum(t ** 4 for t in a)
-    D, dt, kc, apf = p.D_L, p.t_step, p.grad_coeff_c, p.ap
-    num = 1 - dt * mu0 * s1 * (2 * apf * s1 - kc * k3) + 2 * apf * D * k2 * dt
-    return num / (1 + D * dt * kc * k4)
-
-
 @pytest.mark.parametrize("mode,dgb", [((1, 0, 0), 0.0), ((1, 2, 1), 0.0), ((2, 1, 3), 0.3), ((0, 3, 1), 0.5)])
 def test_linear_ch_single_mode_closed_form(mode, dgb):
-    nx, ny, nz = 16, 12, 8
     p = oracle.make_params(t_step=0.01, grad_coeff_c=0.7, D_L=1.3, D_gb=dgb, ap=2.0, con_0_ppt=0.3,
                            mob_phi=0.8)
-    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    th = 2 * np.pi * (mode[0] * x / nx + mode[1] * y / ny + mode[2] * z / nz)
-    c0, eps, a0 = 0.45, 0.01, 0.6
-    con = c0 + eps * np.cos(th)
-    phi = np.full((nz, ny, nx, 1), a0)
-    amp, f = eps, a0
-    nsteps = 6
-    diag = oracle.run(p, phi, con, nsteps)
-    for _ in range(nsteps):
-        mu0 = 2 * (1 - f * f) * dgb
-        amp *= ch_mode_factor(p, mode, (nx, ny, nz), mu0)
-        f = f - p.mob_phi * p.t_step * (f ** 3 - f)
-    assert np.abs(con - (c0 + amp * np.cos(th))).max() < 1e-13
-    assert np.abs(phi - f).max() < 1e-14
-    assert con.mean() == pytest.approx(c0, abs=1e-15)
-    assert abs(amp - eps) > 1e-6  # the mode actually evolved
+    phi, con, ephi, econ, amp = ch_mode_case(p, mode, (16, 12, 8), 6)
+    oracle.run(p, phi, con, 6)
+    assert np.abs(con - econ).max() < 1e-13
+    assert np.abs(phi - ephi).max() < 1e-14
+    assert con.mean() == pytest.approx(0.45, abs=1e-15)
+    assert abs(amp - 0.01) > 1e-6  # the mode actually evolved
 
 
 def test_allen_cahn_var1_harmonics_closed_form():
-    nx, ny, nz = 24, 8, 8
-    m = (2, 1, 0)
     p = oracle.make_params(t_step=0.02, grad_coeff_phi=0.9, mob_phi=1.2)
-    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    th = 2 * np.pi * (m[0] * x / nx + m[1] * y / ny + m[2] * z / nz)
-    a0, eps = 0.3, 0.2
-    phi = (a0 + eps * np.cos(th))[..., None].copy()
-    con = np.full((nz, ny, nx), 0.5)
+    phi, con, expect = ac_harmonics_case(p, (2, 1, 0), (24, 8, 8))
     oracle.run(p, phi, con, 1)
-    L, dt, kap = p.mob_phi, p.t_step, p.grad_coeff_phi
-    psi = [a0 ** 3 + 1.5 * a0 * eps ** 2 - a0, 3 * a0 ** 2 * eps + 0.75 * eps ** 3 - eps,
-           1.5 * a0 * eps ** 2, 0.25 * eps ** 3]
-    coef = [a0, eps, 0.0, 0.0]
-    expect = np.zeros_like(th)
-    for n in range(4):
-        k2 = sum(_axis_a(n * mm, N, False) ** 2 for mm, N in zip(m, (nx, ny, nz)))
-        expect += (coef[n] - L * dt * psi[n]) / (1 + L * kap * dt * k2) * np.cos(n * th)
-    assert np.abs(phi[..., 0] - expect).max() < 1e-14
+    assert np.abs(phi - expect).max() < 1e-14
 
 
 @pytest.mark.parametrize("mode", [(1, 0, 0), (2, 3, 1), (8, 0, 0)])
 def test_gradient_energy_single_mode(mode):
-    nx, ny, nz = 16, 12, 8
     p = oracle.make_params(grad_coeff_phi=0.8, grad_coeff_c=0.6)
-    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    th = 2 * np.pi * (mode[0] * x / nx + mode[1] * y / ny + mode[2] * z / nz)
-    eps_c, eps_p = 0.01, 0.02
-    con = 0.4 + eps_c * np.cos(th)
-    phi = np.stack([0.5 + eps_p * np.cos(th), np.full_like(th, 0.5)], axis=-1)
+    phi, con, expect = single_mode_fint_case(p, mode, (16, 12, 8))
     d = oracle.run(p, phi, con, 1)[0]
-    kp2 = sum(_axis_a(mm, N, True) ** 2 for mm, N in zip(mode, (nx, ny, nz)))
-    P = nx * ny * nz
-    expect = 0.5 * (0.8 * eps_p ** 2 + 0.6 * eps_c ** 2) * kp2 * P / 2
     if mode == (8, 0, 0):  # Nyquist: the spectral gradient is zeroed (src/k_space.f90:51)
-        assert abs(d["f_int"]) < 1e-20
+        assert expect == 0.0 and abs(d["f_int"]) < 1e-20
     else:
         assert d["f_int"] == pytest.approx(expect, rel=1e-12)
 
