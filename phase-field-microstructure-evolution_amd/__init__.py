@@ -83,6 +83,14 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                           f"or `make -C {HERE}`")
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64 (same
+    # soname as /opt/rocm's).  Loaded first, it also satisfies libpf.so's
+    # dependency; loaded after libpf.so, a second runtime would be mapped and
+    # torch.cuda would find no device.  So torch, when installed, goes first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, ip, dp = C.c_void_p, C.c_int, C.c_double
     L.pf_create.argtypes = [C.POINTER(Params), ip, ip, ip, ip, ip, C.POINTER(vp)]

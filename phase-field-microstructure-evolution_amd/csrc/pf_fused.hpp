@@ -127,36 +127,37 @@ __device__ __forceinline__ void nl_point(R* v, long fs, int V, const UPar& u, R&
 //   mobd   : (m - D_mean) per point when mob (input of the mobility chain).
 //   XONLY  : mobility-chain variant for one field: C2R, multiply by mobd, R2C.
 // One job = one (field, line) x transform, done by T = plan_T(M) threads.  The
-// LDS tile is element-major with the jobs as columns (element e of job j at
-// sh[e*JP + j], JP = jobs rounded up to odd), so lanes of a wave, which run
-// consecutive jobs, touch consecutive LDS words in every phase.
-// Dynamic LDS: (M*JP + M) complex + JP reals + 16 doubles.
+// LDS tile is job-major, job j's M complex elements at sh[j*MP + k] with the odd
+// pitch MP = M + 1: lanes of a wave run consecutive jobs (transform phases) or
+// consecutive points (staging, pointwise), so both walk distinct banks, and every
+// address a thread forms is its own row base plus a compile-time offset.
+// Dynamic LDS: (njobs*MP + M) complex + njobs reals + 16 doubles.
 // ---------------------------------------------------------------------------
 template <typename R> constexpr int kXlineMaxThreads = 512;
 // Global loads each thread keeps in flight when staging a line group.
 constexpr int kXlineBatch = 16;
 
-__host__ __device__ constexpr int xline_pitch(int njobs) { return njobs | 1; }
+__host__ __device__ constexpr int xline_row_pitch(int M) { return M | 1; }
 
 template <typename R, int NX, int LINES, bool XONLY>
 __global__ void __launch_bounds__(kXlineMaxThreads<R>)
 k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __restrict__ mobd,
         const c2<R>* __restrict__ tw, double* __restrict__ part, long ngroups, int diag, int mob) {
   constexpr int M = NX / 2, T = plan_T(M), E = plan_E(M);
+  constexpr int MP = xline_row_pitch(M);
   extern __shared__ unsigned char smem[];
   const int F = XONLY ? 1 : g.F;
   const int njobs = F * LINES;
-  const int JP = xline_pitch(njobs);
   c2<R>* sh = reinterpret_cast<c2<R>*>(smem);
-  c2<R>* shw = sh + (long)M * JP;  // W^k, k < M
+  c2<R>* shw = sh + (long)njobs * MP;  // W^k, k < M
   R* shn = reinterpret_cast<R*>(shw + M);
-  double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)M * JP + M) +
-                                                    sizeof(R) * JP) + 15) & ~(size_t)15));
+  double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)njobs * MP + M) +
+                                                    sizeof(R) * njobs) + 15) & ~(size_t)15));
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int job = tid % JP, t = tid / JP;
+  const int job = tid % njobs, t = tid / njobs;
   for (int k = tid; k < M; k += nt) shw[k] = tw[k];
-  const bool act = job < njobs && t < T;
-  c2<R>* col = sh + (act ? job : 0);
+  const bool act = t < T;
+  c2<R>* col = sh + (act ? job : 0) * MP;
   const R scale = (R)u.invP;
   double a_ch = 0.0, a_eta = 0.0, cmax = 0.0, cmin = 1.0, pmin = 1.0, p3 = 0.0;
   c2<R> v[E];
@@ -166,8 +167,8 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const long lg_end = ((lb + 1) * chunk < ngroups) ? (lb + 1) * chunk : ngroups;
   for (long lg = lb * chunk; lg < lg_end; ++lg) {
     const long L0 = lg * LINES;
-    // 1. half lines of every field -> LDS (coalesced along k, transposed into the
-    //    tile), kXlineBatch loads in flight per thread
+    // 1. half lines of every field -> LDS rows (coalesced along k), kXlineBatch
+    //    loads in flight per thread
     for (int i0 = tid; i0 < njobs * M; i0 += kXlineBatch * nt) {
       c2<R> tmp[kXlineBatch];
 #pragma unroll
@@ -184,7 +185,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
         const int i = i0 + q * nt;
         if (i < njobs * M) {
           const int jb = i / M, k = i - jb * M;
-          sh[k * JP + jb] = tmp[q];
+          sh[jb * MP + k] = tmp[q];
         }
       }
     }
@@ -202,24 +203,24 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
         const R a0 = col[0].x, xn = shn[job];
         v[s] = mk<R>(a0 + xn, a0 - xn);
       } else {
-        v[s] = c2r_pre<R>(col[k * JP], col[(M - k) * JP], shw[k]);
+        v[s] = c2r_pre<R>(col[k], col[M - k], shw[k]);
       }
     }
-    // 3. inverse x transform; real pairs z[n] = (x[2n], x[2n+1]) back into the tile
-    rfft<R, M, true, 0>(v, t, act, col, tw, 2, JP);
+    // 3. inverse x transform; real pairs z[n] = (x[2n], x[2n+1]) back into the row
+    rfft<R, M, true, 1>(v, t, act, col, tw, 2);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
-      if (act && rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s) * JP] = v[s];
+      if (act && rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
     __syncthreads();
-    // 4. pointwise: point x of line l is component x%2 of complex row x/2, column
+    // 4. pointwise: point x of line l is component x%2 of element x/2 of row
     //    f*LINES + l
     {
       R* rl = reinterpret_cast<R*>(sh);
-      const long fs = 2L * LINES;  // field stride in reals within a row
+      constexpr long fs = 2L * LINES * MP;  // field stride in reals
       for (int p = tid; p < LINES * NX; p += nt) {
         const int l = p / NX, x = p - l * NX;
-        R* vv = rl + 2L * ((long)(x >> 1) * JP + l) + (x & 1);
+        R* vv = rl + 2L * l * MP + x;
         const long gp = (L0 + l) * NX + x;  // point index within a field
         if (XONLY) {
           vv[0] = vv[0] * mobd[gp];
@@ -235,12 +236,12 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     // 5. forward x transform
 #pragma unroll
     for (int s = 0; s < kInSlots<M>; ++s)
-      if (act && rfft_in_valid<M>(t, s)) v[s] = col[rfft_in_idx<M>(t, s) * JP];
-    rfft<R, M, false, 0>(v, t, act, col, tw, 2, JP);
+      if (act && rfft_in_valid<M>(t, s)) v[s] = col[rfft_in_idx<M>(t, s)];
+    rfft<R, M, false, 1>(v, t, act, col, tw, 2);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
-      if (act && rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s) * JP] = v[s];
+      if (act && rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
     __syncthreads();
     // 6. R2C post-processing and store (coalesced along k)
     const R hs = R(0.5) * scale;
@@ -248,11 +249,11 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       const int jb = i / M, k = i - jb * M;
       const int f = jb / LINES, l = jb - f * LINES;
       const int km = (M - k) % M;
-      wm[(long)f * g.hmain + (L0 + l) * M + k] = r2c_post<R>(sh[k * JP + jb], sh[km * JP + jb], shw[k], hs);
+      wm[(long)f * g.hmain + (L0 + l) * M + k] = r2c_post<R>(sh[jb * MP + k], sh[jb * MP + km], shw[k], hs);
     }
     for (int i = tid; i < njobs; i += nt) {
       const int f = i / LINES, l = i - f * LINES;
-      const c2<R> z0 = sh[i];
+      const c2<R> z0 = sh[i * MP];
       wn[(long)f * g.plane + L0 + l] = mk<R>((z0.x - z0.y) * scale, R(0));
     }
     __syncthreads();

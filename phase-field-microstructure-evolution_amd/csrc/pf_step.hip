@@ -625,24 +625,21 @@ int step_faithful(pf_handle* h, bool diag, long rec) {
 #define PF_GRID_SIZES(X) \
   X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(160) X(192) X(256) X(512)
 
-// Lines per x-line block: small lines are grouped so that a block has work for
-// ~128 points; a second instantiation with twice as many lines is used when it
-// fits (threads, 64 KiB of LDS).
-constexpr int xline_lines(int NX) {
-  int l = 1;
-  while (l * 2 * NX <= 256) l *= 2;
-  return l;
-}
-
+// Lines per x-line block: the smallest of 1, 2, 4, 8 that gives the block's
+// transforms at least two waves of work (F*LINES*T >= 128 threads).  Small
+// blocks keep several of them resident per CU, which is what hides the
+// latency of each block's staging and barriers: at 256^3 x 17 fields one
+// line per block (3 waves, 19 KB of LDS) runs 1.6x faster than two.
+constexpr int kXlineMinThreads = 128;
 template <typename R, int NX, int LINES, bool XONLY>
 bool xline_fits(const Geo& g, long* nt_out, size_t* shb_out) {
   constexpr int M = NX / 2, T = plan_T(M);
   const int F = XONLY ? 1 : g.F;
   if (g.plane % LINES) return false;
   const long jobs = (long)F * LINES;
-  const long JP = xline_pitch((int)jobs);
-  long need = std::max<long>(64, (JP * T + 63) / 64 * 64);
-  const size_t body = sizeof(c2<R>) * ((size_t)M * JP + M) + sizeof(R) * (size_t)JP;
+  const long MP = xline_row_pitch(M);
+  long need = std::max<long>(64, (jobs * T + 63) / 64 * 64);
+  const size_t body = sizeof(c2<R>) * ((size_t)jobs * MP + M) + sizeof(R) * (size_t)jobs;
   const size_t shb = ((body + 15) & ~(size_t)15) + 16 * sizeof(double);
   if (need > kXlineMaxThreads<R> || shb > 65536) return false;
   *nt_out = need;
@@ -670,19 +667,26 @@ int launch_xline(pf_handle* h, c2<R>* wm, c2<R>* wn, R* mobd, double* part, long
                  bool mob) {
   long nt = 0;
   size_t shb = 0;
+  const long F = XONLY ? 1 : h->g.F;
   switch (h->g.nx) {
-#define PF_CASE(N)                                                                                        \
-  case N:                                                                                                 \
-    if (xline_fits<R, N, 2 * xline_lines(N), XONLY>(h->g, &nt, &shb))                                     \
-      return launch_xline_n<R, N, 2 * xline_lines(N), XONLY>(h, wm, wn, mobd, part, grid_out, diag, mob, nt, shb); \
-    if (xline_fits<R, N, xline_lines(N), XONLY>(h->g, &nt, &shb))                                         \
-      return launch_xline_n<R, N, xline_lines(N), XONLY>(h, wm, wn, mobd, part, grid_out, diag, mob, nt, shb); \
-    if (xline_fits<R, N, 1, XONLY>(h->g, &nt, &shb))                                                      \
-      return launch_xline_n<R, N, 1, XONLY>(h, wm, wn, mobd, part, grid_out, diag, mob, nt, shb);        \
-    return fail(PF_ENOTIMPL, "x-line kernel: " + std::to_string(XONLY ? 1 : h->g.F) +                     \
-                                 " fields of length " #N " exceed one block's threads or 64 KiB of LDS");
+#define PF_TRY(N, L)                                                                                     \
+  if (xline_fits<R, N, L, XONLY>(h->g, &nt, &shb))                                                       \
+    return launch_xline_n<R, N, L, XONLY>(h, wm, wn, mobd, part, grid_out, diag, mob, nt, shb);
+#define PF_CASE(N)                                                                                       \
+  case N: {                                                                                              \
+    const long T = plan_T(N / 2);                                                                        \
+    int want = 1;                                                                                        \
+    while (want < 8 && F * want * T < kXlineMinThreads) want *= 2;                                       \
+    if (want >= 8) { PF_TRY(N, 8) }                                                                      \
+    if (want >= 4) { PF_TRY(N, 4) }                                                                      \
+    if (want >= 2) { PF_TRY(N, 2) }                                                                      \
+    PF_TRY(N, 1)                                                                                         \
+    return fail(PF_ENOTIMPL, "x-line kernel: " + std::to_string(F) +                                     \
+                                 " fields of length " #N " exceed one block's threads or 64 KiB of LDS"); \
+  }
     PF_GRID_SIZES(PF_CASE)
 #undef PF_CASE
+#undef PF_TRY
     default: return fail(PF_EINVAL, "unsupported nx");
   }
 }
