@@ -82,8 +82,8 @@ def cpu_baseline(n, var, phi, con, nsteps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--var", type=int, default=16)
     ap.add_argument("--seeds", type=int, default=64)

@@ -129,13 +129,16 @@ __device__ __forceinline__ void nl_point(R* v, long fs, int V, const UPar& u, R&
 // One job = one (field, line) x transform, done by T = plan_T(M) threads.  The
 // LDS tile is job-major, job j's M complex elements at sh[j*MP + k] with the odd
 // pitch MP = M + 1: lanes of a wave run consecutive jobs (transform phases) or
-// consecutive points (staging, pointwise), so both walk distinct banks, and every
-// address a thread forms is its own row base plus a compile-time offset.
-// Dynamic LDS: (njobs*MP + M) complex + njobs reals + 16 doubles.
+// consecutive points (pointwise), so both walk distinct banks, and every address
+// a thread forms is its own row base plus a compile-time offset.  Row njobs is
+// a spare: the threads of the last wave beyond the njobs*T that carry the
+// transforms run the same schedule on it, so no transform slot is predicated.
+// Staging and the final store move whole rows, one per wave at a time.
+// Dynamic LDS: ((njobs+1)*MP + M) complex + (njobs+1) reals + 16 doubles.
 // ---------------------------------------------------------------------------
 template <typename R> constexpr int kXlineMaxThreads = 512;
-// Global loads each thread keeps in flight when staging a line group.
-constexpr int kXlineBatch = 16;
+// Rows each wave keeps in flight when staging a line group.
+constexpr int kXlineRows = 2;
 
 __host__ __device__ constexpr int xline_row_pitch(int M) { return M | 1; }
 
@@ -149,15 +152,22 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const int F = XONLY ? 1 : g.F;
   const int njobs = F * LINES;
   c2<R>* sh = reinterpret_cast<c2<R>*>(smem);
-  c2<R>* shw = sh + (long)njobs * MP;  // W^k, k < M
+  c2<R>* shw = sh + (long)(njobs + 1) * MP;  // W^k, k < M (row njobs: idle threads' row)
   R* shn = reinterpret_cast<R*>(shw + M);
-  double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)njobs * MP + M) +
-                                                    sizeof(R) * njobs) + 15) & ~(size_t)15));
+  double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)(njobs + 1) * MP + M) +
+                                                    sizeof(R) * (njobs + 1)) + 15) & ~(size_t)15));
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int job = tid % njobs, t = tid / njobs;
+  const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  constexpr int KPL = (M + 63) / 64;  // row elements per lane when a wave moves a row
+  const int job = tid % njobs, tq = tid / njobs;
   for (int k = tid; k < M; k += nt) shw[k] = tw[k];
-  const bool act = t < T;
-  c2<R>* col = sh + (act ? job : 0) * MP;
+  // threads beyond the T per job run the same schedule on a spare row, so the
+  // transform phases carry no per-slot predicates
+  const bool act = tq < T;
+  const int t = act ? tq : tq % T;
+  __builtin_assume(t >= 0 && t < T);
+  const int row = act ? job : njobs;
+  c2<R>* col = sh + row * MP;
   const R scale = (R)u.invP;
   double a_ch = 0.0, a_eta = 0.0, cmax = 0.0, cmin = 1.0, pmin = 1.0, p3 = 0.0;
   c2<R> v[E];
@@ -167,25 +177,28 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const long lg_end = ((lb + 1) * chunk < ngroups) ? (lb + 1) * chunk : ngroups;
   for (long lg = lb * chunk; lg < lg_end; ++lg) {
     const long L0 = lg * LINES;
-    // 1. half lines of every field -> LDS rows (coalesced along k), kXlineBatch
-    //    loads in flight per thread
-    for (int i0 = tid; i0 < njobs * M; i0 += kXlineBatch * nt) {
-      c2<R> tmp[kXlineBatch];
+    // 1. half lines of every field -> LDS rows: wave w stages rows w, w + nw, ...
+    //    (coalesced along k), kXlineRows rows per batch in flight
+    for (int jb0 = wv; jb0 < njobs; jb0 += kXlineRows * nw) {
+      c2<R> tmp[kXlineRows][KPL];
 #pragma unroll
-      for (int q = 0; q < kXlineBatch; ++q) {
-        const int i = i0 + q * nt;
-        if (i < njobs * M) {
-          const int jb = i / M, k = i - jb * M;
+      for (int r = 0; r < kXlineRows; ++r) {
+        const int jb = jb0 + r * nw;
+        if (jb < njobs) {
           const int f = jb / LINES, l = jb - f * LINES;
-          tmp[q] = wm[(long)f * g.hmain + (L0 + l) * M + k];
+          const c2<R>* src = wm + (long)f * g.hmain + (L0 + l) * M;
+#pragma unroll
+          for (int kk = 0; kk < KPL; ++kk)
+            if (M % 64 == 0 || lane + 64 * kk < M) tmp[r][kk] = src[lane + 64 * kk];
         }
       }
 #pragma unroll
-      for (int q = 0; q < kXlineBatch; ++q) {
-        const int i = i0 + q * nt;
-        if (i < njobs * M) {
-          const int jb = i / M, k = i - jb * M;
-          sh[jb * MP + k] = tmp[q];
+      for (int r = 0; r < kXlineRows; ++r) {
+        const int jb = jb0 + r * nw;
+        if (jb < njobs) {
+#pragma unroll
+          for (int kk = 0; kk < KPL; ++kk)
+            if (M % 64 == 0 || lane + 64 * kk < M) sh[jb * MP + lane + 64 * kk] = tmp[r][kk];
         }
       }
     }
@@ -197,21 +210,19 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     // 2. C2R pre-processing straight into the transform's input registers
 #pragma unroll
     for (int s = 0; s < kInSlots<M>; ++s) {
-      if (!(act && rfft_in_valid<M>(t, s))) continue;
+      if (!rfft_in_valid<M>(t, s)) continue;
       const int k = rfft_in_idx<M>(t, s);
-      if (k == 0) {
-        const R a0 = col[0].x, xn = shn[job];
-        v[s] = mk<R>(a0 + xn, a0 - xn);
-      } else {
-        v[s] = c2r_pre<R>(col[k], col[M - k], shw[k]);
-      }
+      // k = 0 pairs with the Nyquist entry; col[M] is the row's pad element
+      const c2<R> pk = c2r_pre<R>(col[k], col[M - k], shw[k]);
+      const R a0 = col[0].x, xn = shn[row];
+      v[s] = (k == 0) ? mk<R>(a0 + xn, a0 - xn) : pk;
     }
     // 3. inverse x transform; real pairs z[n] = (x[2n], x[2n+1]) back into the row
-    rfft<R, M, true, 1>(v, t, act, col, tw, 2);
+    rfft<R, M, true, 1>(v, t, true, col, tw, 2);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
-      if (act && rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
+      if (rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
     __syncthreads();
     // 4. pointwise: point x of line l is component x%2 of element x/2 of row
     //    f*LINES + l
@@ -236,20 +247,24 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     // 5. forward x transform
 #pragma unroll
     for (int s = 0; s < kInSlots<M>; ++s)
-      if (act && rfft_in_valid<M>(t, s)) v[s] = col[rfft_in_idx<M>(t, s)];
-    rfft<R, M, false, 1>(v, t, act, col, tw, 2);
+      if (rfft_in_valid<M>(t, s)) v[s] = col[rfft_in_idx<M>(t, s)];
+    rfft<R, M, false, 1>(v, t, true, col, tw, 2);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
-      if (act && rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
+      if (rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
     __syncthreads();
-    // 6. R2C post-processing and store (coalesced along k)
+    // 6. R2C post-processing and store (coalesced along k), one row per wave
     const R hs = R(0.5) * scale;
-    for (int i = tid; i < njobs * M; i += nt) {
-      const int jb = i / M, k = i - jb * M;
+    for (int jb = wv; jb < njobs; jb += nw) {
       const int f = jb / LINES, l = jb - f * LINES;
-      const int km = (M - k) % M;
-      wm[(long)f * g.hmain + (L0 + l) * M + k] = r2c_post<R>(sh[jb * MP + k], sh[jb * MP + km], shw[k], hs);
+      c2<R>* dst = wm + (long)f * g.hmain + (L0 + l) * M;
+      const c2<R>* rw = sh + jb * MP;
+#pragma unroll
+      for (int kk = 0; kk < KPL; ++kk) {
+        const int k = lane + 64 * kk;
+        if (M % 64 == 0 || k < M) dst[k] = r2c_post<R>(rw[k], rw[(M - k) % M], shw[k], hs);
+      }
     }
     for (int i = tid; i < njobs; i += nt) {
       const int f = i / LINES, l = i - f * LINES;
