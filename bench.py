@@ -91,7 +91,22 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (the known-size kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        assert torch.cuda.is_available(), "bench.py needs cuda:0"
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                          "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU decomposition is a design "
               "in DESIGN.md, not a measured path.  Use --gpus 1.", file=sys.stderr)

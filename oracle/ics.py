@@ -138,34 +138,51 @@ def gamma(var, s1, s2):
 
 # ---------------------------------------------------------------------------
 def read_fortran_records(path):
-    """Records of a Fortran unformatted sequential file (4-byte length markers;
-    a negative leading marker continues the record in the next sub-record)."""
+    """Records of a Fortran unformatted sequential file: 4-byte length markers
+    around each (sub)record; the leading marker is negative when the record
+    continues in a further subrecord, the trailing marker negative when the
+    subrecord continues a previous one."""
     recs = []
     with open(path, "rb") as f:
         data = f.read()
     pos = 0
-    cur = b""
+    parts = []
     while pos < len(data):
         (n,) = struct.unpack_from("<i", data, pos)
         ln = abs(n)
-        body = data[pos + 4: pos + 4 + ln]
+        assert pos + 8 + ln <= len(data), "truncated record"
         (tail,) = struct.unpack_from("<i", data, pos + 4 + ln)
         assert abs(tail) == ln, "corrupt record markers"
+        assert (tail < 0) == (len(parts) > 0), "trailing marker sign does not match continuation"
+        parts.append(data[pos + 4: pos + 4 + ln])
         pos += 8 + ln
-        cur += body
         if n >= 0:
-            recs.append(cur)
-            cur = b""
-    assert cur == b"", "truncated continued record"
+            recs.append(parts[0] if len(parts) == 1 else b"".join(parts))
+            parts = []
+    assert not parts, "truncated continued record"
     return recs
 
 
+MAX_SUBRECORD = 2147483639
+
+
 def write_fortran_records(path, recs):
+    """Inverse of read_fortran_records; records longer than MAX_SUBRECORD bytes
+    are split into subrecords (leading marker negative when more follow,
+    trailing marker negative when one precedes)."""
     with open(path, "wb") as f:
         for r in recs:
-            f.write(struct.pack("<i", len(r)))
-            f.write(r)
-            f.write(struct.pack("<i", len(r)))
+            mv = memoryview(r)
+            done = 0
+            while True:
+                sub = min(MAX_SUBRECORD, len(mv) - done)
+                more = done + sub < len(mv)
+                f.write(struct.pack("<i", -sub if more else sub))
+                f.write(mv[done:done + sub])
+                f.write(struct.pack("<i", -sub if done > 0 else sub))
+                done += sub
+                if not more:
+                    break
 
 
 def read_checkpoint(path, nx, ny, nz, var):

@@ -392,6 +392,7 @@ k_zline(Geo g, UPar u, typename KTabOf<R>::T kt, Bank<R> S, Bank<R> W, Bank<R> X
     const R kap = (f == 0) ? (R)u.kc : (R)u.kphi;
     // 2. forward z, update in registers
     rfft<R, NZ, false, TX>(v, t, act, sh + c, tw);
+    R tacc = R(0);  // this tile's Parseval terms, widened once per tile
 #pragma unroll
     for (int s = 0; s < NO; ++s) {
       if (!(act && rfft_out_valid<NZ>(t, s))) continue;
@@ -426,11 +427,12 @@ k_zline(Geo g, UPar u, typename KTabOf<R>::T kt, Bank<R> S, Bank<R> W, Bank<R> X
         sb[ga] = sn;
         if (PARSEVAL) {
           const R wgt = (kx == 0 || kx == g.M) ? R(1) : R(2);
-          acc += (double)((R(0.5) * kap) * wgt * k.kp2 * (sn.x * sn.x + sn.y * sn.y));
+          tacc += (R(0.5) * kap) * wgt * k.kp2 * (sn.x * sn.x + sn.y * sn.y);
         }
       }
       v[s] = sn;
     }
+    if (PARSEVAL) acc += (double)tacc;
     // 3. inverse z (exchange first when the output rows are not the input rows)
     if constexpr (!rfft_io_same<NZ>()) {
       __syncthreads();

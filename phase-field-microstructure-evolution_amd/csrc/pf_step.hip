@@ -28,6 +28,10 @@ static const char* kStageNames[NSTAGE] = {
 constexpr int NB = 256;        // block size of pointwise kernels
 constexpr int NPART_NL = 6;    // f_ch, f_eta, c_max, c_min, phi_min, phi_3
 constexpr long kMaxGrid = 8192;
+// The z-line kernel streams four banks with short (TX-column) row segments;
+// more resident tiles per CU keep more rows in flight (measured 2.5% faster
+// at 32768 than at 8192 blocks, 256^3 x 17 fields).
+constexpr long kZlineMaxGrid = 32768;
 
 using UCoef = UPar;
 
@@ -722,7 +726,7 @@ int launch_zline(pf_handle* h, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y, doubl
   case N: {                                                                                          \
     constexpr int TX = col_tile(N, sizeof(c2<R>));                                                   \
     const long tm = ((long)g.M * g.ny + TX - 1) / TX, tn = ((long)g.ny + TX - 1) / TX;               \
-    const long grid = std::min<long>((tm + tn) * nf, kMaxGrid);                                      \
+    const long grid = std::min<long>((tm + tn) * nf, kZlineMaxGrid);                                \
     const c2<R>* tw = twiddle_table<R>(N);                                                           \
     if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");                                \
     hipLaunchKernelGGL((k_zline<R, N, ROLE, DC64, PARSEVAL>), dim3((unsigned)grid),                  \
@@ -921,7 +925,7 @@ extern "C" int pf_create(const pf_params* p, int nx, int ny, int nz, int var, in
     if ((rc = alloc_reduced<float>(h))) return bail(rc);
   }
   if ((rc = dalloc(h, (void**)&h->part_nl, sizeof(double) * NPART_NL * kMaxGrid))) return bail(rc);
-  if ((rc = dalloc(h, (void**)&h->part_fi, sizeof(double) * 2 * kMaxGrid))) return bail(rc);
+  if ((rc = dalloc(h, (void**)&h->part_fi, sizeof(double) * 2 * std::max(kMaxGrid, kZlineMaxGrid)))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->staging, sizeof(double) * (size_t)g.P * g.F))) return bail(rc);
   h->n_part_nl = grid_for(g.P, NB);
   if ((rc = ensure_diag(h, 64))) return bail(rc);

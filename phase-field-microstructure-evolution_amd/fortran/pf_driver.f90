@@ -6,7 +6,9 @@
 ! is this project's own.
 module pf_driver
   use iso_c_binding
+  use iso_fortran_env, only: int64
   use pf_capi
+  use pf_records
   implicit none
   private
 
@@ -255,16 +257,29 @@ contains
   subroutine read_restart(fname, c, phi, con)
     character(len=*), intent(in) :: fname
     type(run_config), intent(in) :: c
-    real(8), intent(out) :: phi(c%var, c%nx, c%ny, c%nz), con(c%nx, c%ny, c%nz)
-    integer :: u, ist(3), ien(3)
-    open (newunit=u, file=fname, status='old', form='unformatted', action='read')
-    read (u) ist, ien
-    if (any(ist /= 1) .or. ien(1) /= c%nx .or. ien(2) /= c%ny .or. ien(3) /= c%nz) then
-      write (0, *) 'restart file ', trim(fname), ' holds bounds ', ist, ien, &
+    real(8), intent(out), target :: phi(c%var, c%nx, c%ny, c%nz), con(c%nx, c%ny, c%nz)
+    integer, target :: b(6)
+    integer :: u
+    integer(int64) :: pos
+    logical :: ok
+    open (newunit=u, file=fname, status='old', access='stream', form='unformatted', action='read')
+    pos = 1
+    call get_record(u, pos, c_loc(b), 24_int64, c_null_ptr, 0_int64, ok)
+    if (.not. ok) then
+      write (0, *) 'restart file ', trim(fname), ': first record is not ist(3), ien(3)'
+      error stop 1
+    end if
+    if (any(b(1:3) /= 1) .or. b(4) /= c%nx .or. b(5) /= c%ny .or. b(6) /= c%nz) then
+      write (0, *) 'restart file ', trim(fname), ' holds bounds ', b, &
         ' for a ', c%nx, c%ny, c%nz, ' grid'
       error stop 1
     end if
-    read (u) phi, con
+    call get_record(u, pos, c_loc(phi), 8_int64 * size(phi, kind=int64), &
+                    c_loc(con), 8_int64 * size(con, kind=int64), ok)
+    if (.not. ok) then
+      write (0, *) 'restart file ', trim(fname), ': phi/con record is truncated or has the wrong length'
+      error stop 1
+    end if
     close (u)
   end subroutine
 
@@ -330,21 +345,22 @@ contains
     character(len=*), intent(in) :: outdir
     type(run_config), intent(in) :: c
     integer, intent(in) :: step
-    real(8), intent(in) :: phi(c%var, c%nx, c%ny, c%nz), con(c%nx, c%ny, c%nz)
+    real(8), intent(in), target :: phi(c%var, c%nx, c%ny, c%nz), con(c%nx, c%ny, c%nz)
     character(len=512) :: dname
-    integer :: u, ist(3), ien(3), ierr
+    integer, target :: b(6)
+    integer :: u, ierr
     write (dname, '(A, a5, i7.7)') trim(outdir), 'step_', step
     call execute_command_line('mkdir -p '//trim(dname), exitstat=ierr)
     open (newunit=u, file=trim(dname)//'/data_extract_mult.in')
     write (u, *) c%nx, c%ny, c%nz, 1, c%var
     write (u, *) c%dims
     close (u)
-    ist = 1
-    ien = [c%nx, c%ny, c%nz]
-    open (newunit=u, file=trim(dname)//'/data_save.0001', form='unformatted')
-    rewind (u)
-    write (u) ist, ien
-    write (u) phi, con
+    b = [1, 1, 1, c%nx, c%ny, c%nz]
+    open (newunit=u, file=trim(dname)//'/data_save.0001', access='stream', form='unformatted', &
+          status='replace')
+    call put_record(u, c_loc(b), 24_int64, c_null_ptr, 0_int64)
+    call put_record(u, c_loc(phi), 8_int64 * size(phi, kind=int64), &
+                    c_loc(con), 8_int64 * size(con, kind=int64))
     flush (u)
     close (u)
   end subroutine

@@ -14,12 +14,14 @@ from driver_util import DRIVER, cfg_of, run_driver, write_input
 pytestmark = pytest.mark.skipif(not os.path.exists(DRIVER), reason="var_diff_hip.x not built")
 
 
-def _dry(tmp_path, **kw):
+def _dry(tmp_path, check_state=True, **kw):
     inp = tmp_path / "input.txt"
     p = write_input(inp, **kw)
     out = tmp_path / "out"
     r = run_driver(str(tmp_path), str(inp), str(out), env={"PF_DRY_RUN": "1"})
     assert r.returncode == 0, r.stderr
+    if not check_state:
+        return p, None, None, out, r
     ist, ien, phi, con = ics.read_checkpoint(str(out / "step_0000000" / "data_save.0001"),
                                               p["Nx"], p["Ny"], p["Nz"], p["var"])
     assert list(ist) == [1, 1, 1] and list(ien) == [p["Nx"], p["Ny"], p["Nz"]]
@@ -79,3 +81,42 @@ def test_checkpoint_reader_rejects_truncation(tmp_path):
     open(tmp_path / "g", "wb").write(data[:-6])
     with pytest.raises(Exception):
         ics.read_fortran_records(str(tmp_path / "g"))
+
+
+def test_record_over_2gib_uses_subrecords(tmp_path):
+    """At 256^3 x 17 fields the phi+con record is 2.28 GB: it must be split into
+    subrecords (a single 4-byte marker cannot hold its length).  Restart from a
+    file written by the restatement, write it back, and compare byte for byte."""
+    n, var = 256, 16
+    rng = np.random.default_rng(3)
+    phi0 = rng.random((n, n, n, var))
+    con0 = rng.random((n, n, n))
+    src = tmp_path / "data_rerun.0001"
+    ics.write_restart(str(src), phi0, con0)
+    del phi0, con0
+    _, _, _, out, _ = _dry(tmp_path, nrun=2, Nx=n, Ny=n, Nz=n, var=var, check_state=False)
+    dst = out / "step_0000000" / "data_save.0001"
+    assert os.path.getsize(dst) == os.path.getsize(src) == 24 + 8 + 8 * 2 + 8 * n ** 3 * (var + 1)
+    with open(src, "rb") as a, open(dst, "rb") as b:
+        while True:
+            x, y = a.read(1 << 26), b.read(1 << 26)
+            assert x == y
+            if not x:
+                break
+    with open(dst, "rb") as f:
+        f.seek(32)
+        import struct
+        assert struct.unpack("<i", f.read(4))[0] == -ics.MAX_SUBRECORD
+
+
+def test_truncated_restart_is_rejected(tmp_path):
+    phi0 = np.zeros((8, 8, 8, 3))
+    con0 = np.full((8, 8, 8), 0.3)
+    src = tmp_path / "data_rerun.0001"
+    ics.write_restart(str(src), phi0, con0)
+    data = open(src, "rb").read()
+    open(src, "wb").write(data[:-100])
+    inp = tmp_path / "input.txt"
+    write_input(inp, nrun=2, Nx=8, Ny=8, Nz=8, var=3)
+    r = run_driver(str(tmp_path), str(inp), str(tmp_path / "out"), env={"PF_DRY_RUN": "1"})
+    assert r.returncode != 0 and "truncated" in r.stderr
