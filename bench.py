@@ -63,6 +63,31 @@ def roofline_of(stages, steps):
     return name, avg_ms, achieved, st["bytes"]
 
 
+# Kernel behind each timed stage of the FP32 step at 256^3 (names as rocprofv3 prints them).
+STAGE_KERNELS = {
+    "nonlinear": ["pf::k_xline<float, 256, 1, false>"],
+    "y_fwd": ["pf::k_ypass<float, 256, false>"],
+    "kspace_update": ["pf::k_zline<float, 256, 0, true, false>", "pf::k_zline<float, 256, 0, true, true>"],
+    "y_inv": ["pf::k_ypass<float, 256, true>"],
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fused_kernels_256x17_fp32.md")
+
+
+def measured_traffic(stage, n, var):
+    """HBM bytes per launch of the stage's kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), or None when that
+    summary does not cover this workload."""
+    if n != 256 or var != 16 or stage not in STAGE_KERNELS or not os.path.exists(TRAFFIC_FILE):
+        return None
+    txt = open(TRAFFIC_FILE).read()
+    try:
+        blob = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
+        vals = [blob["kernels"][k]["traffic_bytes"] for k in STAGE_KERNELS[stage]]
+    except (IndexError, KeyError, ValueError):
+        return None
+    return sum(vals) / len(vals)
+
+
 def cpu_baseline(n, var, phi, con, nsteps):
     import oracle
 
@@ -108,8 +133,8 @@ def main():
                           "launches": 5}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; the multi-GPU decomposition is a design "
-              "in DESIGN.md, not a measured path.  Use --gpus 1.", file=sys.stderr)
+        print("bench.py: this build runs on one GPU; the multi-GPU (z-slab) decomposition is "
+              "designed, not built or measured.  Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -173,7 +198,8 @@ def main():
                    "parallelism": "single GPU"},
         "field_updates_per_s": gpups * F,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name, n, var),
+                     "stage": name, "kernel": " / ".join(STAGE_KERNELS.get(name, [name])),
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": kbytes},
         "step_model": {"model_bytes_per_step": model_bytes,
                        "model_GBs": model_bytes / (ms_step * 1e-3) / 1e9,
