@@ -104,6 +104,35 @@ def cpu_baseline(n, var, phi, con, nsteps):
                        f"({dt:.1f} s), oracle/pf_oracle.c with {threads} OpenMP threads")
 
 
+def accuracy(pf, n, var, phi, con, args):
+    """FP32-mixed against an FP64 form on the benchmark workload: per-field max
+    and RMS absolute difference at every checkpoint (one JSON line)."""
+    prm = workload_params(pf)
+    rows = []
+    with pf.Simulation(prm, n, n, n, var, "fp32_mixed") as a, \
+            pf.Simulation(prm, n, n, n, var, args.accuracy_ref) as b:
+        a.put_state(phi, con)
+        b.put_state(phi, con)
+        done = 0
+        while done < args.accuracy_steps:
+            k = min(args.accuracy_every, args.accuracy_steps - done)
+            a.step(k)
+            b.step(k)
+            done += k
+            pa, ca = a.get_state()
+            pb, cb = b.get_state()
+            dmax, drms = [float(np.abs(ca - cb).max())], [float(np.sqrt(np.mean((ca - cb) ** 2)))]
+            for i in range(var):
+                d = pa[..., i] - pb[..., i]
+                dmax.append(float(np.abs(d).max()))
+                drms.append(float(np.sqrt(np.mean(d * d))))
+            rows.append({"step": done, "max": dmax, "rms": drms,
+                         "c_mean_fp32": float(ca.mean()), "c_mean_ref": float(cb.mean())})
+            log(f"accuracy step {done}: worst field max {max(dmax):.3e}, rms {max(drms):.3e}")
+    print(json.dumps({"accuracy": f"fp32_mixed vs {args.accuracy_ref}", "grid": [n, n, n], "var": var,
+                      "fields": "c, eta_1..eta_V", "checkpoints": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +145,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="instead of timing: run FP32-mixed and FP64 (--accuracy-ref) side by side "
+                         "and print per-field max / RMS differences every --accuracy-every steps")
+    ap.add_argument("--accuracy-steps", type=int, default=1000)
+    ap.add_argument("--accuracy-every", type=int, default=100)
+    ap.add_argument("--accuracy-ref", default="fp64_faithful", choices=["fp64_faithful", "fp64_reduced"])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes) and exit")
@@ -146,6 +181,10 @@ def main():
     n, var = args.size, args.var
     log(f"building Voronoi IC {n}^3 x {var + 1} fields, {args.seeds} seeds")
     phi, con = pf.ic_voronoi(n, n, n, var, args.seeds, -1, 0.4, 0.02)
+
+    if args.accuracy:
+        accuracy(pf, n, var, phi, con, args)
+        return
 
     log(f"fp32_mixed: {args.warmup} warmup + {args.steps} timed steps")
     dt, stages = run_mode(pf, torch, "fp32_mixed", n, var, phi, con, args.steps, args.warmup)

@@ -136,7 +136,10 @@ __device__ __forceinline__ void nl_point(R* v, long fs, int V, const UPar& u, R&
 // Staging and the final store move whole rows, one per wave at a time.
 // Dynamic LDS: ((njobs+1)*MP + M) complex + (njobs+1) reals + 16 doubles.
 // ---------------------------------------------------------------------------
-template <typename R> constexpr int kXlineMaxThreads = 512;
+// Threads per x-line block: FP32 blocks may use 1024 threads (the kernel fits in
+// 128 VGPRs), which a 512^3 grid with 37 fields needs (37 transforms x 16
+// threads); FP64 blocks stop at 512.
+template <typename R> constexpr int kXlineMaxThreads = sizeof(R) == 4 ? 1024 : 512;
 // Rows each wave keeps in flight when staging a line group.
 constexpr int kXlineRows = 2;
 
@@ -337,21 +340,31 @@ k_ypass(c2<R>* __restrict__ main, c2<R>* __restrict__ nyq, long M, long npanels,
 // The spectrum values a thread holds after the forward transform are updated in
 // registers (S read and written at the same rows); when the transform's output
 // rows differ from its input rows an LDS exchange precedes the inverse.
+// The wavenumbers and the update coefficients are evaluated in FP64 in both
+// precisions: in FP32 a rounded coefficient A = 1/(1 + dt M kappa k^4) would
+// bias every step the same way, and over 1000 steps of a slowly decaying mode
+// that bias grows linearly (measured 1.5e-4 max error in c at 256^3 x 17,
+// against rounding of the products alone, which averages out).
 // ---------------------------------------------------------------------------
 enum { ROLE_REG = 0, ROLE_GHAT = 1, ROLE_CMOB = 2 };
 
-template <typename R> __device__ __forceinline__ R recip(R x) { return R(1) / x; }
-template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+// 1/x in FP64 for x in [1, 2^60): hardware reciprocal estimate and two Newton
+// steps (each doubles the correct bits), within a few ulp of the quotient.
+__device__ __forceinline__ double recip_fast(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * (2.0 - x * r);
+  return r * (2.0 - x * r);
+}
 
-// Minimum waves per SIMD asked of the register allocator: in FP32 the Parseval
-// variant otherwise takes 172 VGPRs (2 waves) where 168 (3 waves) suffice with
-// two spills, 12% faster at 256^3 x 17 fields.
-template <typename R> constexpr int kZlineWaves = sizeof(R) == 4 ? 3 : 2;
+// Minimum waves per SIMD asked of the register allocator (256^3 x 17 fields):
+// the FP32 update fits 3 waves without spills; its Parseval variant needs more
+// registers and runs best at 2 (3 would spill 14 VGPRs: 1.39 against 1.17 ms).
+template <typename R, bool PARSEVAL> constexpr int kZlineWaves = (sizeof(R) == 4 && !PARSEVAL) ? 3 : 2;
 
 template <typename R, int NZ, int ROLE, bool DC64, bool PARSEVAL>
 __global__ void __launch_bounds__(col_tile(NZ, sizeof(c2<R>)) * plan_T(NZ))
-__attribute__((amdgpu_waves_per_eu(kZlineWaves<R>)))
-k_zline(Geo g, UPar u, typename KTabOf<R>::T kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
+__attribute__((amdgpu_waves_per_eu(kZlineWaves<R, PARSEVAL>)))
+k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
         double* __restrict__ dc64, double* __restrict__ part, const c2<R>* __restrict__ tw, int f0, int nf,
         long tm, long tn) {
   constexpr int TX = col_tile(NZ, sizeof(c2<R>));
@@ -361,8 +374,8 @@ k_zline(Geo g, UPar u, typename KTabOf<R>::T kt, Bank<R> S, Bank<R> W, Bank<R> X
   const long tpf = tm + tn;
   const long total = (long)nf * tpf;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const R Dk = (R)((u.D * u.dt) * u.kc), Ddt = (R)(u.D * u.dt), Lkd = (R)((u.L * u.kphi) * u.dt);
-  const R Ldt = (R)(u.L * u.dt), dtR = (R)u.dt, kcR = (R)u.kc;
+  const double Dk = (u.D * u.dt) * u.kc, Ddt = u.D * u.dt, Lkd = (u.L * u.kphi) * u.dt;
+  const double Ldt = u.L * u.dt, dtQ = u.dt, kcQ = u.kc;
   double acc = 0.0;
   c2<R> v[E];
   c2<R> sv[NO];
@@ -394,8 +407,8 @@ k_zline(Geo g, UPar u, typename KTabOf<R>::T kt, Bank<R> S, Bank<R> W, Bank<R> X
     const long col = col0 + c;
     const int kx = isn ? g.M : (int)(col % g.M);
     const int ky = isn ? (int)col : (int)(col / g.M);
-    const KCol<R> kc = kcol<R>(kt, kx, ky);
-    const R kap = (f == 0) ? (R)u.kc : (R)u.kphi;
+    const KCol<double> kc = kcol<double>(kt, kx, ky);
+    const double kap = (f == 0) ? u.kc : u.kphi;
     // 2. forward z, update in registers
     rfft<R, NZ, false, TX>(v, t, act, sh + c, tw);
     R tacc = R(0);  // this tile's Parseval terms, widened once per tile
@@ -403,28 +416,43 @@ k_zline(Geo g, UPar u, typename KTabOf<R>::T kt, Bank<R> S, Bank<R> W, Bank<R> X
     for (int s = 0; s < NO; ++s) {
       if (!(act && rfft_out_valid<NZ>(t, s))) continue;
       const int e = rfft_out_idx<NZ>(t, s);
-      const KCoefR<R> k = kcoef_col<R>(kt, kc, e);
+      const KCoefR<double> k = kcoef_col<double>(kt, kc, e);
       const c2<R> w = v[s];
       const long ga = (long)e * Wd;
       c2<R> sn;
       if constexpr (ROLE == ROLE_GHAT) {
         const c2<R> ch = sv[s];
         xb[ga] = w;  // f^
-        sn = mk<R>(-(k.s1 * w.y) + kcR * ch.y * k.k3, k.s1 * w.x - kcR * ch.x * k.k3);
+        sn = mk<R>((R)(-(k.s1 * w.y) + kcQ * ch.y * k.k3), (R)(k.s1 * w.x - kcQ * ch.x * k.k3));
       } else {
         const c2<R> s0 = sv[s];
-        if (f == 0) {
-          const R A = recip<R>(R(1) + Dk * k.k4), B = (Ddt * k.k2) * A;
+        // A = 1/den multiplies the whole state each step; B (and Cc) the step's
+        // increment
+        const double den = (f == 0) ? 1.0 + Dk * k.k4 : 1.0 - Lkd * k.k2;
+        if constexpr (sizeof(R) == 8) {
+          const double A = 1.0 / den;
+          const double B = (f == 0) ? (Ddt * k.k2) * A : -Ldt * A;
           if constexpr (ROLE == ROLE_CMOB) {
             const c2<R> fh = xv[s];
-            const R Cc = (dtR * k.s1) * A;
+            const double Cc = (dtQ * k.s1) * A;
             sn = mk<R>(A * s0.x + B * fh.x - Cc * w.y, A * s0.y + B * fh.y + Cc * w.x);
           } else {
             sn = mk<R>(A * s0.x + B * w.x, A * s0.y + B * w.y);
           }
         } else {
-          const R A = recip<R>(R(1) - Lkd * k.k2), B = -Ldt * A;
-          sn = mk<R>(A * s0.x + B * w.x, A * s0.y + B * w.y);
+          // FP32 state: A carried as an unevaluated sum hi + lo of two floats, so
+          // the only per-step error on the state is the rounding of the products
+          const double A = recip_fast(den);
+          const float Ah = (float)A, Al = (float)(A - (double)Ah);
+          const float B = (float)((f == 0) ? (Ddt * k.k2) * A : -Ldt * A);
+          float ix = B * (float)w.x, iy = B * (float)w.y;
+          if constexpr (ROLE == ROLE_CMOB) {
+            const c2<R> fh = xv[s];
+            const float Cc = (float)((dtQ * k.s1) * A);
+            ix = B * fh.x - Cc * w.y;
+            iy = B * fh.y + Cc * w.x;
+          }
+          sn = mk<R>(fmaf(Ah, s0.x, fmaf(Al, s0.x, ix)), fmaf(Ah, s0.y, fmaf(Al, s0.y, iy)));
         }
         if (DC64 && kx == 0 && ky == 0 && e == 0) {
           if (f != 0) dc64[f] = dc64[f] - (u.L * u.dt) * (double)w.x;
@@ -433,7 +461,7 @@ k_zline(Geo g, UPar u, typename KTabOf<R>::T kt, Bank<R> S, Bank<R> W, Bank<R> X
         sb[ga] = sn;
         if (PARSEVAL) {
           const R wgt = (kx == 0 || kx == g.M) ? R(1) : R(2);
-          tacc += (R(0.5) * kap) * wgt * k.kp2 * (sn.x * sn.x + sn.y * sn.y);
+          tacc += (R)((0.5 * kap) * k.kp2) * wgt * (sn.x * sn.x + sn.y * sn.y);
         }
       }
       v[s] = sn;

@@ -635,6 +635,10 @@ int step_faithful(pf_handle* h, bool diag, long rec) {
 // latency of each block's staging and barriers: at 256^3 x 17 fields one
 // line per block (3 waves, 19 KB of LDS) runs 1.6x faster than two.
 constexpr int kXlineMinThreads = 128;
+// LDS one x-line block may take (gfx950: 160 KiB per workgroup); above 64 KiB
+// the kernel is opted in with hipFuncAttributeMaxDynamicSharedMemorySize.
+constexpr size_t kXlineMaxLds = 160 * 1024;
+
 template <typename R, int NX, int LINES, bool XONLY>
 bool xline_fits(const Geo& g, long* nt_out, size_t* shb_out) {
   constexpr int M = NX / 2, T = plan_T(M);
@@ -645,7 +649,7 @@ bool xline_fits(const Geo& g, long* nt_out, size_t* shb_out) {
   long need = std::max<long>(64, (jobs * T + 63) / 64 * 64);
   const size_t body = sizeof(c2<R>) * ((size_t)(jobs + 1) * MP + M) + sizeof(R) * (size_t)(jobs + 1);
   const size_t shb = ((body + 15) & ~(size_t)15) + 16 * sizeof(double);
-  if (need > kXlineMaxThreads<R> || shb > 65536) return false;
+  if (need > kXlineMaxThreads<R> || shb > kXlineMaxLds) return false;
   *nt_out = need;
   *shb_out = shb;
   return true;
@@ -660,6 +664,10 @@ int launch_xline_n(pf_handle* h, c2<R>* wm, c2<R>* wn, R* mobd, double* part, lo
   if (grid >= 8) grid -= grid % 8;
   const c2<R>* tw = twiddle_table<R>(NX);
   if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  if (shb > 65536 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_xline<R, NX, LINES, XONLY>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shb) != hipSuccess)
+    return fail(PF_EHIP, "x-line kernel: " + std::to_string(shb) + " bytes of LDS refused");
   hipLaunchKernelGGL((k_xline<R, NX, LINES, XONLY>), dim3((unsigned)grid), dim3((unsigned)nt), shb, h->s, g, h->u,
                      wm, wn, mobd, tw, part, ngroups, (int)diag, (int)mob);
   if (grid_out) *grid_out = grid;
@@ -686,7 +694,7 @@ int launch_xline(pf_handle* h, c2<R>* wm, c2<R>* wn, R* mobd, double* part, long
     if (want >= 2) { PF_TRY(N, 2) }                                                                      \
     PF_TRY(N, 1)                                                                                         \
     return fail(PF_ENOTIMPL, "x-line kernel: " + std::to_string(F) +                                     \
-                                 " fields of length " #N " exceed one block's threads or 64 KiB of LDS"); \
+                                 " fields of length " #N " exceed one block's threads or LDS"); \
   }
     PF_GRID_SIZES(PF_CASE)
 #undef PF_CASE
@@ -730,7 +738,7 @@ int launch_zline(pf_handle* h, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y, doubl
     const c2<R>* tw = twiddle_table<R>(N);                                                           \
     if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");                                \
     hipLaunchKernelGGL((k_zline<R, N, ROLE, DC64, PARSEVAL>), dim3((unsigned)grid),                  \
-                       dim3(TX * plan_T(N)), 0, h->s, g, h->u, ktab<R>(h), S, W, X, Y, h->dc64, part, tw, \
+                       dim3(TX * plan_T(N)), 0, h->s, g, h->u, h->kt, S, W, X, Y, h->dc64, part, tw,      \
                        f0, nf, tm, tn);                                                              \
     if (grid_out) *grid_out = grid;                                                                  \
     return 0;                                                                                        \

@@ -17,17 +17,24 @@ template <typename R> struct KTabOf;
 template <> struct KTabOf<double> { using T = KTab; };
 template <> struct KTabOf<float> { using T = KTabF; };
 
-// Per-column wavenumbers (kx, ky fixed) and the kz-dependent completion, in
-// precision R: s1 = kf_sum/i, k2 = k_sq, k3 = k_cube/i, k4 = k_4, kp2 = |kf|^2.
+// Per-column (kx, ky fixed) partial sums of the wavenumber combinations and the
+// kz-dependent completion, in precision R: s1 = kf_sum/i, k2 = k_sq, k3 =
+// k_cube/i, k4 = k_4, kp2 = |kf|^2.  Each completion adds the z term last, the
+// order of the per-point expressions of src/var_diff.f90:932-935, 978-981, 1019,
+// so hoisting the (x, y) parts changes no bit.
 template <typename R> struct KCol {
-  R ax, axp, ay, ayp, x2, y2, x4, y4;
+  R s1, k2, k3, k4, kp2;  // x + y parts
 };
 template <typename R, typename KT>
 __device__ __forceinline__ KCol<R> kcol(const KT& t, int kx, int ky) {
+  const R ax = t.ax[kx], axp = t.axp[kx], ay = t.ay[ky], ayp = t.ayp[ky];
+  const R x2 = ax * ax, y2 = ay * ay;
   KCol<R> c;
-  c.ax = t.ax[kx]; c.axp = t.axp[kx]; c.ay = t.ay[ky]; c.ayp = t.ayp[ky];
-  c.x2 = c.ax * c.ax; c.y2 = c.ay * c.ay;
-  c.x4 = (c.x2 * c.ax) * c.ax; c.y4 = (c.y2 * c.ay) * c.ay;
+  c.s1 = axp + ayp;
+  c.k2 = -x2 + -y2;
+  c.k3 = -(axp * x2) + -(ayp * y2);
+  c.k4 = (x2 * ax) * ax + (y2 * ay) * ay;
+  c.kp2 = axp * axp + ayp * ayp;
   return c;
 }
 template <typename R> struct KCoefR {
@@ -38,11 +45,11 @@ __device__ __forceinline__ KCoefR<R> kcoef_col(const KT& t, const KCol<R>& c, in
   const R az = t.az[kz], azp = t.azp[kz];
   const R z2 = az * az;
   KCoefR<R> k;
-  k.s1 = (c.axp + c.ayp) + azp;
-  k.k2 = (-c.x2 + -c.y2) + -z2;
-  k.k3 = (-(c.axp * c.x2) + -(c.ayp * c.y2)) + -(azp * z2);
-  k.k4 = (c.x4 + c.y4) + (z2 * az) * az;
-  k.kp2 = (c.axp * c.axp + c.ayp * c.ayp) + azp * azp;
+  k.s1 = c.s1 + azp;
+  k.k2 = c.k2 + -z2;
+  k.k3 = c.k3 + -(azp * z2);
+  k.k4 = c.k4 + (z2 * az) * az;
+  k.kp2 = c.kp2 + azp * azp;
   return k;
 }
 

@@ -97,3 +97,38 @@ def test_split_run_equals_continuous_run(pf, mode):
         assert np.array_equal(pa, pb) and np.array_equal(ca, cb)
     else:  # the carried spectrum is rebuilt from the real state at put_state
         assert np.abs(pa - pb).max() < 1e-13 and np.abs(ca - cb).max() < 1e-13
+
+
+def test_fp32_long_x_lines_many_fields(pf, oracle):
+    """x lines of 512 points with 37 fields: one 1024-thread block per line and
+    80 KiB of LDS (the 512^3 x 37 configuration's x-line shape, on a 512 x 16 x 16
+    grid the oracle can follow)."""
+    _compare(pf, oracle, (512, 16, 16), 36, 40, 6, "fp32_mixed", dict(VOR, D_gb=0.0), 1e-4, 1e-4)
+
+
+def test_fp64_reduced_too_many_fields_is_an_error(pf):
+    """FP64 x-line blocks stop at 512 threads: 37 fields of 512-point lines need 592."""
+    phi, con = pf.ic_voronoi(512, 16, 16, 36, 40)
+    with pytest.raises(pf.PfError) as e:
+        with pf.Simulation(pf.make_params(**VOR), 512, 16, 16, 36, "fp64_reduced") as sim:
+            sim.put_state(phi, con)
+            sim.step(1)
+    assert e.value.code == pf.PF_ENOTIMPL
+
+
+@pytest.mark.parametrize("mode", ["fp64_faithful", "fp64_reduced"])
+def test_fp64_500_steps_vs_oracle(pf, oracle, mode):
+    """500 steps of a 16^3 x 5-field polycrystal with D_gb != 0: per-field max
+    relative difference to the oracle <= 1e-12."""
+    n, var = 16, 4
+    prm = dict(VOR, D_gb=0.3)
+    phi, con = pf.ic_voronoi(n, n, n, var, 6)
+    with pf.Simulation(pf.make_params(**prm), n, n, n, var, mode) as sim:
+        sim.put_state(phi, con)
+        sim.step(500, diag_every=0)
+        gphi, gcon = sim.get_state()
+    ophi, ocon = phi.copy(), con.copy()
+    oracle.run(oracle.make_params(**prm), ophi, ocon, 500)
+    errs = [_rel(gphi[..., i], ophi[..., i]) for i in range(var)] + [_rel(gcon, ocon)]
+    print(mode, "500-step per-field rel diff", ["%.1e" % e for e in errs])
+    assert max(errs) <= 1e-12
