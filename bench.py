@@ -133,6 +133,36 @@ def accuracy(pf, n, var, phi, con, args):
                       "fields": "c, eta_1..eta_V", "checkpoints": rows}), flush=True)
 
 
+def boundary(pf, torch, n, var, phi, con):
+    """Host-buffer entry points on the benchmark workload, PCIe included (one JSON line)."""
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    res = {"grid": [n, n, n], "var": var}
+    state_bytes = phi.nbytes + con.nbytes
+    for mode in ("fp32_mixed", "fp64_reduced", "fp64_faithful"):
+        with pf.Simulation(workload_params(pf), n, n, n, var, mode) as sim:
+            put = timed(lambda: sim.put_state(phi, con))
+            get = timed(lambda: sim.get_state())
+        res[mode] = {"put_state_ms": put, "get_state_ms": get, "host_state_bytes": state_bytes,
+                     "put_GBs": state_bytes / put / 1e6, "get_GBs": state_bytes / get / 1e6}
+        log(f"{mode}: put_state {put:.1f} ms, get_state {get:.1f} ms ({state_bytes / 1e9:.2f} GB)")
+    x = np.ascontiguousarray(con)
+    ft = timed(lambda: pf.f_trans(x))
+    h = pf.f_trans(x)
+    it = timed(lambda: pf.inv_trans(h, n))
+    res["f_trans_ms"], res["inv_trans_ms"] = ft, it
+    res["f_trans_GPUPS"], res["inv_trans_GPUPS"] = n ** 3 / ft * 1e3, n ** 3 / it * 1e3
+    log(f"f_trans {ft:.1f} ms, inv_trans {it:.1f} ms per {n}^3 field (host FP64 in and out)")
+    print(json.dumps({"boundary": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +181,9 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=1000)
     ap.add_argument("--accuracy-every", type=int, default=100)
     ap.add_argument("--accuracy-ref", default="fp64_faithful", choices=["fp64_faithful", "fp64_reduced"])
+    ap.add_argument("--boundary", action="store_true",
+                    help="instead of timing steps: time the host-buffer entry points (state upload/"
+                         "download, f_trans/inv_trans drop-ins), PCIe included")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes) and exit")
@@ -184,6 +217,9 @@ def main():
 
     if args.accuracy:
         accuracy(pf, n, var, phi, con, args)
+        return
+    if args.boundary:
+        boundary(pf, torch, n, var, phi, con)
         return
 
     log(f"fp32_mixed: {args.warmup} warmup + {args.steps} timed steps")

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pf.h"
@@ -834,6 +835,47 @@ template <typename R> int alloc_reduced(pf_handle* h) {
   return rc;
 }
 
+constexpr int kMeanChunks = 64;
+
+struct Neumaier {
+  double s = 0.0, c = 0.0;
+  void add(double x) {
+    const double t = s + x;
+    if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+    s = t;
+  }
+  double value() const { return s + c; }
+};
+
+std::vector<double> host_means(const double* phi, const double* con, long P, int var) {
+  const int F = var + 1;
+  std::vector<Neumaier> part((size_t)kMeanChunks * F);
+  auto run = [&](int ch) {
+    const long t0 = P * ch / kMeanChunks, t1 = P * (ch + 1) / kMeanChunks;
+    Neumaier* acc = &part[(size_t)ch * F];
+    for (long t = t0; t < t1; ++t) {
+      acc[0].add(con[t]);
+      const double* p = phi + t * var;
+      for (int f = 1; f < F; ++f) acc[f].add(p[f - 1]);
+    }
+  };
+  const int nth = (int)std::max(1u, std::min<unsigned>(16, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int w = 0; w < nth; ++w)
+    th.emplace_back([&, w] { for (int ch = w; ch < kMeanChunks; ch += nth) run(ch); });
+  for (auto& t : th) t.join();
+  std::vector<double> dc(F);
+  for (int f = 0; f < F; ++f) {
+    Neumaier tot;
+    for (int ch = 0; ch < kMeanChunks; ++ch) {
+      tot.add(part[(size_t)ch * F + f].s);
+      tot.add(part[(size_t)ch * F + f].c);
+    }
+    dc[f] = tot.value() / (double)(float)P;
+  }
+  return dc;
+}
+
 template <typename R> int put_reduced(pf_handle* h, const double* phi, const double* con) {
   const Geo& g = h->g;
   // W <- real state; S <- F(W) with f_trans normalisation
@@ -845,19 +887,11 @@ template <typename R> int put_reduced(pf_handle* h, const double* phi, const dou
   // W <- x-half lines: (1/nx) R2C along x of the real state == z,y-inverse of S
   rc = x_r2c<R>(g.nx, (c2<R>*)h->W_main, (c2<R>*)h->W_nyq, (long)g.ny * g.nz * g.F, (R)(1.0 / (double)g.nx), h->s);
   if (rc) return rc;
-  // FP64 means (k = 0 modes), Neumaier-compensated
-  std::vector<double> dc(g.F, 0.0);
-  for (int f = 0; f < g.F; ++f) {
-    double sm = 0.0, cc = 0.0;
-    for (long t = 0; t < g.P; ++t) {
-      const double x = (f == 0) ? con[t] : phi[t * g.var + (f - 1)];
-      const double tt = sm + x;
-      if (std::fabs(sm) >= std::fabs(x)) cc += (sm - tt) + x; else cc += (x - tt) + sm;
-      sm = tt;
-    }
-    dc[f] = (sm + cc) / (double)(float)g.P;
-  }
-  PF_HIP(hipMemcpyAsync(h->dc64, dc.data(), sizeof(double) * g.F, hipMemcpyHostToDevice, h->s));
+  // FP64 means (k = 0 modes), Neumaier-compensated: one pass over the host
+  // arrays in kMeanChunks fixed chunks (threads take whole chunks), chunk
+  // partials combined in chunk order, so the result does not depend on the
+  // thread count
+  std::vector<double> dc = host_means(phi, con, g.P, g.var);  PF_HIP(hipMemcpyAsync(h->dc64, dc.data(), sizeof(double) * g.F, hipMemcpyHostToDevice, h->s));
   PF_HIP(hipStreamSynchronize(h->s));
   return 0;
 }
