@@ -116,7 +116,13 @@ contains
     do n = 1, c%nx * c%ny * c%nz
       read (u, *) ii, jj, kk, ivar
       if (ii >= 1 .and. ii <= c%nx .and. jj >= 1 .and. jj <= c%ny .and. &
-          kk >= 1 .and. kk <= c%nz) phi(ivar, ii, jj, kk) = 1.d0
+          kk >= 1 .and. kk <= c%nz) then
+        if (ivar < 1 .or. ivar > c%var) then
+          write (0, *) trim(fname), ': grain index ', ivar, ' outside 1..', c%var
+          error stop 1
+        end if
+        phi(ivar, ii, jj, kk) = 1.d0
+      end if
     end do
     close (u)
   end subroutine

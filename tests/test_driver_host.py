@@ -120,3 +120,50 @@ def test_truncated_restart_is_rejected(tmp_path):
     write_input(inp, nrun=2, Nx=8, Ny=8, Nz=8, var=3)
     r = run_driver(str(tmp_path), str(inp), str(tmp_path / "out"), env={"PF_DRY_RUN": "1"})
     assert r.returncode != 0 and "truncated" in r.stderr
+
+
+def test_polycryst_ic_from_grain_map(tmp_path):
+    """start = 'polycryst': one 'i j k ivar' line per point, any order
+    (src/var_diff.f90:232-249)."""
+    nx, ny, nz, var = 12, 8, 10, 5
+    rng = np.random.default_rng(4)
+    ivar = rng.integers(1, var + 1, size=(nz, ny, nx))
+    pts = [(i + 1, j + 1, k + 1, int(ivar[k, j, i])) for k in range(nz) for j in range(ny) for i in range(nx)]
+    order = rng.permutation(len(pts))
+    gm = tmp_path / "grains.txt"
+    gm.write_text("".join("%d %d %d %d\n" % pts[o] for o in order))
+    inp = tmp_path / "input.txt"
+    p = write_input(inp, Nx=nx, Ny=ny, Nz=nz, var=var, start="polycryst")
+    out = tmp_path / "out"
+    r = run_driver(str(tmp_path), str(inp), str(out), env={"PF_DRY_RUN": "1", "PF_POLYCRYST_FILE": str(gm)})
+    assert r.returncode == 0, r.stderr
+    _, _, phi, con = ics.read_checkpoint(str(out / "step_0000000" / "data_save.0001"), nx, ny, nz, var)
+    expect = np.zeros((nz, ny, nx, var))
+    for (i, j, k, v) in pts:
+        expect[k - 1, j - 1, i - 1, v - 1] = 1.0
+    assert np.array_equal(phi, expect) and np.all(con == p["c_mat"])
+    # a grain index beyond var is an error, not a stray write
+    gm.write_text("1 1 1 %d\n" % (var + 1) + "".join("%d %d %d %d\n" % q for q in pts[1:]))
+    r = run_driver(str(tmp_path), str(inp), str(out), env={"PF_DRY_RUN": "1", "PF_POLYCRYST_FILE": str(gm)})
+    assert r.returncode != 0 and "outside" in r.stderr
+
+
+def test_checkpoint_read_by_fortran_runtime(tmp_path):
+    """The checkpoint read back with plain Fortran sequential reads (the
+    post-processor's way), compiled here with amdflang."""
+    import shutil
+    import subprocess
+    fc = shutil.which("amdflang") or "/opt/rocm/llvm/bin/amdflang"
+    if not os.path.exists(fc):
+        pytest.skip("amdflang not available")
+    exe = tmp_path / "read_checkpoint"
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran", "read_checkpoint.f90")
+    subprocess.run([fc, "-O1", src, "-o", str(exe)], check=True, capture_output=True, cwd=str(tmp_path))
+    p, phi, con, out, _ = _dry(tmp_path, Nx=24, Ny=16, Nz=8, var=4, grn_rad=4, num_ppt=2, start="circle")
+    r = subprocess.run([str(exe), str(out / "step_0000000" / "data_save.0001"), "24", "16", "8", "4"],
+                       check=True, capture_output=True, text=True)
+    lines = r.stdout.split("\n")
+    assert [int(x) for x in lines[0].split()] == [1, 1, 1, 24, 16, 8]
+    s = [float(x) for x in lines[1].split()]
+    assert s[0] == pytest.approx(phi.sum(), rel=1e-13) and s[1] == pytest.approx(con.sum(), rel=1e-13)
+    assert s[2] == pytest.approx((phi[..., 3] * con).sum(), rel=1e-13)
