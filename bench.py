@@ -163,6 +163,34 @@ def boundary(pf, torch, n, var, phi, con):
     print(json.dumps({"boundary": res}), flush=True)
 
 
+def ensemble(pf, torch, args):
+    """Parameter-sweep ensemble: aggregate grid-point-updates/s of N simulations
+    advanced together (pf_ensemble_step), after 20 untimed steps."""
+    import pf_amd.runconfig as rc
+
+    members = rc.ensemble_members(pf, n=args.ensemble, var=8)
+    pts = sum(m["grid"] ** 3 for m in members)
+    log(f"ensemble: {len(members)} simulations, {pts / 1e6:.1f} M points in total")
+    sims = rc.run_ensemble(pf, members, 20)
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pf.ensemble_step(sims, args.ensemble_steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    finally:
+        for s in sims:
+            s.close()
+    gpups = pts * args.ensemble_steps / dt
+    log(f"ensemble: {dt / args.ensemble_steps * 1e3:.3f} ms per ensemble step, {gpups:.3e} GPUPS")
+    print(json.dumps({"metric": "grid-point-updates/s, parameter-sweep ensemble", "value": gpups,
+                      "unit": "grid-point-updates/s", "members": len(members), "points": pts,
+                      "steps": args.ensemble_steps, "ms_per_step": dt / args.ensemble_steps * 1e3,
+                      "dtype": "f32", "config": {"workload": f"{len(members)} Voronoi polycrystals "
+                      "(16 seeds each), grids 64/96/128/160/192^3 in turn, c + 8 eta, dt=0.005, kappa~U(0.25,2), "
+                      "M~U(0.5,2), L~U(0.5,2) from one ran_2 stream"}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +212,10 @@ def main():
     ap.add_argument("--boundary", action="store_true",
                     help="instead of timing steps: time the host-buffer entry points (state upload/"
                          "download, f_trans/inv_trans drop-ins), PCIe included")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="instead of the single grid: N independent FP32 simulations, grids cycling "
+                         "64..192^3, var 8, parameters from one ran_2 stream, advanced together")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes) and exit")
@@ -211,6 +243,9 @@ def main():
     pf = __graft_entry__.load_package()
     pf.lib()
     assert torch.cuda.is_available(), "bench.py needs cuda:0"
+    if args.ensemble:
+        ensemble(pf, torch, args)
+        return
     n, var = args.size, args.var
     log(f"building Voronoi IC {n}^3 x {var + 1} fields, {args.seeds} seeds")
     phi, con = pf.ic_voronoi(n, n, n, var, args.seeds, -1, 0.4, 0.02)

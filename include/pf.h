@@ -78,6 +78,13 @@ int pf_get_state(pf_handle* h, double* phi, double* con);
 int pf_step(pf_handle* h, int nsteps, int diag_every, pf_diag* out);
 /* Diagnostics of the current state (the start of the next step). */
 int pf_diagnose(pf_handle* h, pf_diag* out);
+/* Ensemble of independent simulations (the reference runs one var_diff.x job
+ * per parameter set, tests/scaling_tests.py:53-69 and tests/run_phase_sim.py:
+ * 112-150): advances each of the n handles by nsteps steps, no diagnostics.
+ * Steps are enqueued round-robin on the handles' own streams, so the GPU runs
+ * the simulations concurrently; returns once every stream has finished.  The
+ * handles must be distinct. */
+int pf_ensemble_step(pf_handle* const* hs, int n, int nsteps);
 /* Per-stage timing with HIP events on the handle's stream: enable/disable and
  * read the accumulated milliseconds and launch counts of up to n stages. */
 int pf_set_timing(pf_handle* h, int on);

@@ -67,6 +67,7 @@ EXPORTED = (
     "pf_set_timing", "pf_stage_times", "pf_stage_name", "pf_stage_bytes", "pf_stream",
     "pf_fft_r2c", "pf_fft_c2r", "pf_f_trans", "pf_inv_trans", "pf_ran2_init", "pf_ran2",
     "pf_k_space", "pf_gamma", "pf_ic_random", "pf_ic_voronoi", "pf_last_error", "pf_version",
+    "pf_ensemble_step",
 )
 
 
@@ -119,6 +120,7 @@ def lib():
     L.pf_ic_random.argtypes = [ip, ip, ip, ip, ip, dp, dp, dp, _dp, _dp]
     L.pf_ic_voronoi.argtypes = [ip, ip, ip, ip, ip, ip, dp, dp, _dp, _dp]
     L.pf_last_error.restype = C.c_char_p
+    L.pf_ensemble_step.argtypes = [C.POINTER(vp), ip, ip]
     _lib = L
     return L
 
@@ -195,6 +197,10 @@ class Simulation:
         if out is None:
             return []
         return [out[i].as_dict() for i in range(nsteps) if out[i].valid]
+
+    @property
+    def handle(self):
+        return self._h
 
     def diagnose(self) -> dict:
         d = Diag()
@@ -292,3 +298,11 @@ def fft_r2c_device(nx, ny, nz, batch, prec, d_in: int, d_out: int, layout=LAYOUT
 def fft_c2r_device(nx, ny, nz, batch, prec, d_in: int, d_out: int, layout=LAYOUT_XFAST,
                    stream=None) -> None:
     _check(lib().pf_fft_c2r(nx, ny, nz, batch, prec, d_in, d_out, layout, stream))
+
+
+def ensemble_step(sims, nsteps: int) -> None:
+    """Advance independent simulations together (pf_ensemble_step): each step is
+    enqueued on every simulation's stream in turn, so the GPU runs them
+    concurrently; returns when all have finished."""
+    hs = (C.c_void_p * len(sims))(*[s.handle for s in sims])
+    _check(lib().pf_ensemble_step(hs, len(sims), nsteps))

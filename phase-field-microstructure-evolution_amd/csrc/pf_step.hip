@@ -1037,6 +1037,32 @@ extern "C" int pf_get_state(pf_handle* h, double* phi, double* con) {
   return 0;
 }
 
+extern "C" int pf_ensemble_step(pf_handle* const* hs, int n, int nsteps) {
+  if (n < 0 || nsteps < 0 || (n > 0 && !hs)) return fail(PF_EINVAL, "bad ensemble arguments");
+  for (int i = 0; i < n; ++i) {
+    if (!hs[i]) return fail(PF_EINVAL, "null handle in ensemble");
+    for (int j = 0; j < i; ++j)
+      if (hs[j] == hs[i]) return fail(PF_EINVAL, "a handle appears twice in the ensemble");
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    for (int i = 0; i < n; ++i) {
+      pf_handle* h = hs[i];
+      int rc;
+      if (h->mode == PF_MODE_FP64_FAITHFUL) rc = step_faithful(h, false, 0);
+      else if (h->mode == PF_MODE_FP64_REDUCED) rc = step_reduced<double>(h, false, 0, false);
+      else rc = step_reduced<float>(h, false, 0, false);
+      if (rc) return rc;
+      h->step += 1;
+    }
+  }
+  PF_HIP(hipGetLastError());
+  for (int i = 0; i < n; ++i) {
+    PF_HIP(hipStreamSynchronize(hs[i]->s));
+    tflush(hs[i]);
+  }
+  return 0;
+}
+
 extern "C" int pf_step(pf_handle* h, int nsteps, int diag_every, pf_diag* out) {
   if (!h) return fail(PF_EINVAL, "null handle");
   if (nsteps < 0) return fail(PF_EINVAL, "nsteps < 0");

@@ -153,3 +153,33 @@ def test_fp32_error_after_1000_steps(pf):
     errs = [float(np.abs(ca - cb).max())] + [float(np.abs(pa[..., i] - pb[..., i]).max()) for i in range(var)]
     print("fp32 vs fp64 after 1000 steps, per-field max", ["%.1e" % e for e in errs])
     assert max(errs) <= 3e-5
+
+
+def test_ensemble_step_equals_individual_steps(pf):
+    """pf_ensemble_step interleaves the members' steps on their own streams; each
+    member's result is bit-identical to stepping it alone."""
+    shapes = [(16, 16, 16, 3, "fp32_mixed"), (24, 16, 32, 4, "fp64_reduced"), (32, 32, 32, 3, "fp64_faithful")]
+    prm = pf.make_params(**dict(VOR, D_gb=0.2))
+    ics_ = [pf.ic_voronoi(nx, ny, nz, var, 5, -(i + 1)) for i, (nx, ny, nz, var, _) in enumerate(shapes)]
+    alone = []
+    for (nx, ny, nz, var, mode), (phi, con) in zip(shapes, ics_):
+        with pf.Simulation(prm, nx, ny, nz, var, mode) as s:
+            s.put_state(phi, con)
+            s.step(7)
+            alone.append(s.get_state())
+    sims = []
+    try:
+        for (nx, ny, nz, var, mode), (phi, con) in zip(shapes, ics_):
+            s = pf.Simulation(prm, nx, ny, nz, var, mode)
+            s.put_state(phi, con)
+            sims.append(s)
+        pf.ensemble_step(sims, 7)
+        for s, (pa, ca) in zip(sims, alone):
+            pb, cb = s.get_state()
+            assert np.array_equal(pa, pb) and np.array_equal(ca, cb)
+    finally:
+        for s in sims:
+            s.close()
+    with pf.Simulation(prm, 16, 16, 16, 3, "fp32_mixed") as s:
+        with pytest.raises(pf.PfError):
+            pf.ensemble_step([s, s], 1)  # a handle may appear once
