@@ -133,6 +133,18 @@ int pf_ic_random(int nx, int ny, int nz, int var, int iseed, double c0, double c
 int pf_ic_voronoi(int nx, int ny, int nz, int var, int nseeds, int iseed, double c0,
                   double c_amp, double* phi, double* con);
 
+/* Post-processing of one state as src/paraview.f90:188-313 does per voxel
+ * (phi in the reference's (var, nx, ny, nz) layout; outputs nx*ny*nz values, x
+ * fastest): phi_sq = sum_i phi_i^2; var_num = the last i with phi_i > 0.8 (0 if
+ * none); phi_3 = phi_3 (0 when var < 3); boundary = 1 where more than one phi_i
+ * exceeds threshold; grad_phi_3 = |central-difference gradient of phi_3| on
+ * interior voxels (0 on the faces, which the reference leaves unset).
+ * counts[0] = boundary voxels with phi_var > threshold, counts[1] = boundary
+ * voxels.  Runs on the GPU; buffers are host memory. */
+int pf_analyze_state(const double* phi, int nx, int ny, int nz, int var, double threshold,
+                     double* phi_sq, double* phi_3, double* grad_phi_3, double* boundary,
+                     int* var_num, long* counts);
+
 const char* pf_last_error(void);
 int pf_version(void);
 

@@ -203,3 +203,36 @@ def write_restart(path, phi, con):
     r1 = np.array([1, 1, 1, nx, ny, nz], dtype="<i4").tobytes()
     r2 = np.ascontiguousarray(phi, dtype="<f8").tobytes() + np.ascontiguousarray(con, dtype="<f8").tobytes()
     write_fortran_records(path, [r1, r2])
+
+
+def analyze(phi, threshold):
+    """The per-voxel quantities and counts of src/paraview.f90:188-313 for one
+    state phi (nz, ny, nx, var); arrays returned flat, x fastest.  The gradient
+    faces (left unset by the reference) are 0 here."""
+    nz, ny, nx, var = phi.shape
+    sq = (phi * phi).sum(axis=-1)
+    above = (phi > threshold).sum(axis=-1)
+    big = phi > 0.8
+    vmax = np.where(big.any(axis=-1), var - np.argmax(big[..., ::-1], axis=-1), 0)
+    bnd = above > 1
+    p3 = phi[..., 2] if var >= 3 else np.zeros((nz, ny, nx))
+    g = np.zeros((nz, ny, nx))
+    c = p3
+    gx = (c[1:-1, 1:-1, 2:] - c[1:-1, 1:-1, :-2]) / 2.0
+    gy = (c[1:-1, 2:, 1:-1] - c[1:-1, :-2, 1:-1]) / 2.0
+    gz = (c[2:, 1:-1, 1:-1] - c[:-2, 1:-1, 1:-1]) / 2.0
+    g[1:-1, 1:-1, 1:-1] = np.sqrt(gx * gx + gy * gy + gz * gz)
+    counts = (int((bnd & (phi[..., var - 1] > threshold)).sum()), int(bnd.sum()))
+    return dict(phi_sq=sq.ravel(), var_num=vmax.ravel(), phi_3=p3.ravel(), grad_phi_3=g.ravel(),
+                boundary=bnd.astype(float).ravel(), counts=counts)
+
+
+def read_vtk_values(path):
+    """Values of an ASCII STRUCTURED_POINTS file written by the post-processor."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    assert lines[0] == "# vtk DataFile Version 3.0" and lines[9] == "LOOKUP_TABLE default"
+    dims = [int(x) for x in lines[4].split()[1:]]
+    vals = np.array(" ".join(lines[10:]).split(), dtype=float)
+    assert vals.size == dims[0] * dims[1] * dims[2]
+    return dims, vals

@@ -67,7 +67,7 @@ EXPORTED = (
     "pf_set_timing", "pf_stage_times", "pf_stage_name", "pf_stage_bytes", "pf_stream",
     "pf_fft_r2c", "pf_fft_c2r", "pf_f_trans", "pf_inv_trans", "pf_ran2_init", "pf_ran2",
     "pf_k_space", "pf_gamma", "pf_ic_random", "pf_ic_voronoi", "pf_last_error", "pf_version",
-    "pf_ensemble_step",
+    "pf_ensemble_step", "pf_analyze_state",
 )
 
 
@@ -121,6 +121,9 @@ def lib():
     L.pf_ic_voronoi.argtypes = [ip, ip, ip, ip, ip, ip, dp, dp, _dp, _dp]
     L.pf_last_error.restype = C.c_char_p
     L.pf_ensemble_step.argtypes = [C.POINTER(vp), ip, ip]
+    L.pf_analyze_state.argtypes = [_dp, ip, ip, ip, ip, dp, _dp, _dp, _dp, _dp,
+                                   np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
+                                   np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
     _lib = L
     return L
 
@@ -306,3 +309,18 @@ def ensemble_step(sims, nsteps: int) -> None:
     concurrently; returns when all have finished."""
     hs = (C.c_void_p * len(sims))(*[s.handle for s in sims])
     _check(lib().pf_ensemble_step(hs, len(sims), nsteps))
+
+
+def analyze_state(phi: np.ndarray, threshold: float) -> dict:
+    """src/paraview.f90:188-313 for one state phi (nz, ny, nx, var) on the GPU:
+    flat per-voxel arrays (x fastest) and (precipitate boundary voxels, boundary voxels)."""
+    nz, ny, nx, var = phi.shape
+    P = nx * ny * nz
+    out = {k: np.empty(P) for k in ("phi_sq", "phi_3", "grad_phi_3", "boundary")}
+    vn = np.empty(P, dtype=np.int32)
+    counts = np.zeros(2, dtype=np.int64)
+    _check(lib().pf_analyze_state(np.ascontiguousarray(phi, dtype=np.float64), nx, ny, nz, var, threshold,
+                                  out["phi_sq"], out["phi_3"], out["grad_phi_3"], out["boundary"], vn, counts))
+    out["var_num"] = vn
+    out["counts"] = (int(counts[0]), int(counts[1]))
+    return out

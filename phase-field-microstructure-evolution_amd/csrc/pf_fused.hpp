@@ -356,10 +356,9 @@ __device__ __forceinline__ double recip_fast(double x) {
   return r * (2.0 - x * r);
 }
 
-// Minimum waves per SIMD asked of the register allocator (256^3 x 17 fields):
-// the FP32 update fits 3 waves without spills; its Parseval variant needs more
-// registers and runs best at 2 (3 would spill 14 VGPRs: 1.39 against 1.17 ms).
-template <typename R, bool PARSEVAL> constexpr int kZlineWaves = (sizeof(R) == 4 && !PARSEVAL) ? 3 : 2;
+// Minimum waves per SIMD asked of the register allocator: FP32 fits 3 waves
+// (158 / 168 VGPRs with and without the Parseval sums, no spills), FP64 2.
+template <typename R, bool PARSEVAL> constexpr int kZlineWaves = sizeof(R) == 4 ? 3 : 2;
 
 template <typename R, int NZ, int ROLE, bool DC64, bool PARSEVAL>
 __global__ void __launch_bounds__(col_tile(NZ, sizeof(c2<R>)) * plan_T(NZ))
@@ -408,15 +407,24 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
     const int kx = isn ? g.M : (int)(col % g.M);
     const int ky = isn ? (int)col : (int)(col / g.M);
     const KCol<double> kc = kcol<double>(kt, kx, ky);
+    // The z-table entries a thread needs are the same for every tile; an opaque
+    // zero offset keeps the compiler from hoisting all of them (2 doubles per
+    // slot) into registers for the whole loop, re-reading them from L1 instead.
+    KTab ktz = kt;
+    {
+      long zo = 0;
+      asm volatile("" : "+s"(zo));
+      ktz.az += zo;
+      ktz.azp += zo;
+    }
     const double kap = (f == 0) ? u.kc : u.kphi;
     // 2. forward z, update in registers
     rfft<R, NZ, false, TX>(v, t, act, sh + c, tw);
-    R tacc = R(0);  // this tile's Parseval terms, widened once per tile
 #pragma unroll
     for (int s = 0; s < NO; ++s) {
       if (!(act && rfft_out_valid<NZ>(t, s))) continue;
       const int e = rfft_out_idx<NZ>(t, s);
-      const KCoefR<double> k = kcoef_col<double>(kt, kc, e);
+      const KCoefR<double> k = kcoef_col<double>(ktz, kc, e);
       const c2<R> w = v[s];
       const long ga = (long)e * Wd;
       c2<R> sn;
@@ -459,14 +467,27 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
           sn = mk<R>((R)dc64[f], R(0));
         }
         sb[ga] = sn;
-        if (PARSEVAL) {
-          const R wgt = (kx == 0 || kx == g.M) ? R(1) : R(2);
-          tacc += (R)((0.5 * kap) * k.kp2) * wgt * (sn.x * sn.x + sn.y * sn.y);
-        }
+
       }
       v[s] = sn;
     }
-    if (PARSEVAL) acc += (double)tacc;
+    if (PARSEVAL && ROLE != ROLE_GHAT) {
+      // Parseval terms of the new spectrum, in a pass of their own after the
+      // update (fewer registers live at once): |kf'|^2 = (x,y part) + azp^2,
+      // the column part applied once per tile
+      R pa1 = R(0), pa2 = R(0);
+#pragma unroll
+      for (int s = 0; s < NO; ++s) {
+        if (!(act && rfft_out_valid<NZ>(t, s))) continue;
+        const R m2 = v[s].x * v[s].x + v[s].y * v[s].y;
+        const int e = rfft_out_idx<NZ>(t, s);
+        const R zp = (2 * e == NZ) ? R(0) : (R)ktz.az[e];  // a'_z: a_z zeroed at Nyquist
+        pa1 += m2;
+        pa2 += zp * zp * m2;
+      }
+      const double wgt = (kx == 0 || kx == g.M) ? 1.0 : 2.0;
+      acc += (0.5 * kap) * wgt * (kc.kp2 * (double)pa1 + (double)pa2);
+    }
     // 3. inverse z (exchange first when the output rows are not the input rows)
     if constexpr (!rfft_io_same<NZ>()) {
       __syncthreads();

@@ -34,7 +34,7 @@ module pf_capi
 
   public :: pf_create, pf_destroy, pf_put_state, pf_get_state, pf_step, pf_diagnose
   public :: pf_set_timing, pf_stage_times, pf_stage_name, pf_f_trans, pf_inv_trans
-  public :: pf_ran2_init, pf_ran2, pf_k_space, pf_gamma, pf_last_error
+  public :: pf_ran2_init, pf_ran2, pf_k_space, pf_gamma, pf_last_error, pf_analyze_state
   public :: pf_error_message, pf_check, ran_2
 
   interface
@@ -117,6 +117,17 @@ module pf_capi
       integer(c_int), value :: var
       real(c_double), value :: s1, s2
       real(c_double), intent(out) :: g(*)
+    end function
+    integer(c_int) function pf_analyze_state(phi, nx, ny, nz, var, threshold, phi_sq, phi_3, &
+                                             grad_phi_3, boundary, var_num, counts) &
+      bind(C, name="pf_analyze_state")
+      import :: c_int, c_double, c_long
+      real(c_double), intent(in) :: phi(*)
+      integer(c_int), value :: nx, ny, nz, var
+      real(c_double), value :: threshold
+      real(c_double), intent(out) :: phi_sq(*), phi_3(*), grad_phi_3(*), boundary(*)
+      integer(c_int), intent(out) :: var_num(*)
+      integer(c_long), intent(out) :: counts(2)
     end function
     type(c_ptr) function pf_last_error() bind(C, name="pf_last_error")
       import :: c_ptr
