@@ -183,3 +183,20 @@ def test_ensemble_step_equals_individual_steps(pf):
     with pf.Simulation(prm, 16, 16, 16, 3, "fp32_mixed") as s:
         with pytest.raises(pf.PfError):
             pf.ensemble_step([s, s], 1)  # a handle may appear once
+
+
+# Every supported axis length through the fused kernels (x lines, y panels, z
+# columns), small enough in the other axes for the oracle.
+AXIS_SHAPES = [(8, 12, 16), (12, 8, 24), (48, 16, 8), (96, 8, 12), (160, 8, 8), (192, 12, 8),
+               (64, 48, 8), (8, 96, 12), (8, 12, 160), (16, 192, 8), (128, 8, 12), (8, 8, 256)]
+
+
+@pytest.mark.parametrize("shape", AXIS_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("mode,tol_s,tol_e", [("fp64_reduced", 1e-11, 1e-10), ("fp32_mixed", 1e-4, 1e-4)])
+def test_axis_lengths_vs_oracle(pf, oracle, shape, mode, tol_s, tol_e):
+    _compare(pf, oracle, shape, 4, 6, 4, mode, dict(VOR, D_gb=0.25), tol_s, tol_e)
+
+
+@pytest.mark.parametrize("shape", [(160, 8, 8), (12, 96, 8), (8, 12, 192)], ids=lambda s: "x".join(map(str, s)))
+def test_axis_lengths_faithful_vs_oracle(pf, oracle, shape):
+    _compare(pf, oracle, shape, 3, 5, 4, "fp64_faithful", dict(VOR, D_gb=0.25), 1e-11, 1e-10)
