@@ -23,12 +23,15 @@ namespace pf {
 
 constexpr int kBlock = 256;
 
-// columns per tile of a column pass: 16..64 columns, block of ~256 threads,
-// exchange tile <= 64 KiB
+// columns per tile of a column pass: a power of two from 16 to 64 (every row
+// segment a whole number of 128-byte lines), block of <= ~256 threads, exchange
+// tile <= 64 KiB.  (Lengths 96, 160, 192 have T = 6, 10, 12 threads per column;
+// 256/T would give 42, 25, 21 columns and misaligned segments, which ran the
+// 192^3 z-line at 1.8 TB/s.)
 __host__ __device__ constexpr int col_tile(int N, int cbytes) {
-  int tx = kBlock / plan_T(N);
-  if (tx < 16) tx = 16;
-  if (tx > 64) tx = 64;
+  const int want = kBlock / plan_T(N);
+  int tx = 16;
+  while (tx * 2 <= want && tx < 64) tx *= 2;
   while (tx > 1 && N * tx * cbytes > 65536) tx /= 2;
   return tx;
 }
