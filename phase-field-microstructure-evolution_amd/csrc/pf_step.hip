@@ -466,6 +466,21 @@ struct pf_handle {
   long st_cnt[NSTAGE] = {};
 };
 
+// Makes the handle's device current for the duration of an entry point and
+// restores the caller's device afterwards: kernels, streams and allocations of
+// a handle all belong to the device it was created on.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
 namespace {
 
 template <typename R> typename KTabOf<R>::T ktab(const pf_handle* h);
@@ -701,6 +716,23 @@ int launch_xline(pf_handle* h, c2<R>* wm, c2<R>* wn, R* mobd, double* part, long
 #undef PF_CASE
 #undef PF_TRY
     default: return fail(PF_EINVAL, "unsupported nx");
+  }
+}
+
+// Whether some x-line block shape fits this grid (checked at pf_create, so
+// that an unsupported combination fails there rather than at the first step).
+template <typename R, bool XONLY>
+bool xline_supported(const Geo& g) {
+  long nt = 0;
+  size_t shb = 0;
+  switch (g.nx) {
+#define PF_CASE(N)                                                                                 \
+  case N:                                                                                          \
+    return xline_fits<R, N, 8, XONLY>(g, &nt, &shb) || xline_fits<R, N, 4, XONLY>(g, &nt, &shb) ||   \
+           xline_fits<R, N, 2, XONLY>(g, &nt, &shb) || xline_fits<R, N, 1, XONLY>(g, &nt, &shb);
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return false;
   }
 }
 
@@ -962,8 +994,14 @@ extern "C" int pf_create(const pf_params* p, int nx, int ny, int nz, int var, in
         (rc = alloc_bank<double>(h, h->fpsi, var)) || (rc = dalloc(h, (void**)&h->fmob, sizeof(double) * g.P)))
       return bail(rc);
   } else if (mode == PF_MODE_FP64_REDUCED) {
+    if (!xline_supported<double, false>(g))
+      return bail(fail(PF_ENOTIMPL, "FP64 reduced form: " + std::to_string(g.F) + " fields of " +
+                                        std::to_string(nx) + "-point x lines exceed one block's threads or LDS"));
     if ((rc = alloc_reduced<double>(h))) return bail(rc);
   } else {
+    if (!xline_supported<float, false>(g))
+      return bail(fail(PF_ENOTIMPL, "FP32-mixed form: " + std::to_string(g.F) + " fields of " +
+                                        std::to_string(nx) + "-point x lines exceed one block's threads or LDS"));
     if ((rc = alloc_reduced<float>(h))) return bail(rc);
   }
   if ((rc = dalloc(h, (void**)&h->part_nl, sizeof(double) * NPART_NL * kMaxGrid))) return bail(rc);
@@ -977,6 +1015,7 @@ extern "C" int pf_create(const pf_params* p, int nx, int ny, int nz, int var, in
 
 extern "C" int pf_destroy(pf_handle* h) {
   if (!h) return 0;
+  DeviceGuard dg(h->dev);
   if (h->s) (void)hipStreamSynchronize(h->s);
   for (void* p : h->allocs) (void)hipFree(p);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
@@ -987,6 +1026,7 @@ extern "C" int pf_destroy(pf_handle* h) {
 
 extern "C" int pf_put_state(pf_handle* h, const double* phi, const double* con) {
   if (!h || !phi || !con) return fail(PF_EINVAL, "null argument");
+  DeviceGuard dg(h->dev);
   const Geo& g = h->g;
   PF_HIP(hipMemcpyAsync(h->staging, phi, sizeof(double) * (size_t)g.P * g.var, hipMemcpyHostToDevice, h->s));
   PF_HIP(hipMemcpyAsync(h->staging + (size_t)g.P * g.var, con, sizeof(double) * (size_t)g.P,
@@ -1014,6 +1054,7 @@ template <typename R> int real_state_reduced(pf_handle* h) {
 
 extern "C" int pf_get_state(pf_handle* h, double* phi, double* con) {
   if (!h || !phi || !con) return fail(PF_EINVAL, "null argument");
+  DeviceGuard dg(h->dev);
   const Geo& g = h->g;
   double* sphi = h->staging;
   double* scon = h->staging + (size_t)g.var * g.P;
@@ -1043,7 +1084,9 @@ extern "C" int pf_ensemble_step(pf_handle* const* hs, int n, int nsteps) {
     if (!hs[i]) return fail(PF_EINVAL, "null handle in ensemble");
     for (int j = 0; j < i; ++j)
       if (hs[j] == hs[i]) return fail(PF_EINVAL, "a handle appears twice in the ensemble");
+    if (hs[i]->dev != hs[0]->dev) return fail(PF_EINVAL, "ensemble handles are on different devices");
   }
+  DeviceGuard dg(n > 0 ? hs[0]->dev : 0);
   for (int s = 0; s < nsteps; ++s) {
     for (int i = 0; i < n; ++i) {
       pf_handle* h = hs[i];
@@ -1065,6 +1108,7 @@ extern "C" int pf_ensemble_step(pf_handle* const* hs, int n, int nsteps) {
 
 extern "C" int pf_step(pf_handle* h, int nsteps, int diag_every, pf_diag* out) {
   if (!h) return fail(PF_EINVAL, "null handle");
+  DeviceGuard dg(h->dev);
   if (nsteps < 0) return fail(PF_EINVAL, "nsteps < 0");
   int rc = 0;
   if (out && (rc = ensure_diag(h, std::max(1, nsteps)))) return rc;
@@ -1110,6 +1154,7 @@ extern "C" int pf_step(pf_handle* h, int nsteps, int diag_every, pf_diag* out) {
 
 extern "C" int pf_diagnose(pf_handle* h, pf_diag* out) {
   if (!h || !out) return fail(PF_EINVAL, "null argument");
+  DeviceGuard dg(h->dev);
   const Geo& g = h->g;
   hipStream_t s = h->s;
   int rc = ensure_diag(h, 1);
