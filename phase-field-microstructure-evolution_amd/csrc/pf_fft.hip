@@ -2,6 +2,7 @@
 // whole-transform drivers, layout packing and the transform entry points of the
 // C-ABI (pf_fft_r2c / pf_fft_c2r on device buffers, pf_f_trans / pf_inv_trans
 // on host buffers with the reference's f_trans / inv_trans semantics).
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -104,6 +105,28 @@ int x_c2r(int nx, c2<R>* data, const c2<R>* nyq, long nlines, hipStream_t s) {
 }
 
 template <typename R>
+int x_c2r_sumsq(int nx, const c2<R>* data, const c2<R>* nyq, long nlines, long lines_per_field, double w0,
+                double w1, double* part, long max_blocks, long* nblocks, hipStream_t s) {
+  const c2<R>* tw = twiddle_table<R>(nx);
+  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  switch (nx) {
+#define PF_CASE(N)                                                                                   \
+  case N: {                                                                                          \
+    constexpr int LPB = lines_per_block(N / 2);                                                      \
+    const long grid = std::max<long>(1, std::min<long>((nlines + LPB - 1) / LPB, max_blocks));       \
+    hipLaunchKernelGGL((k_x_c2r_sumsq<R, N>), dim3((unsigned)grid), dim3(LPB * plan_T(N / 2)), 0, s, \
+                       data, nyq, nlines, lines_per_field, w0, w1, tw, part);                        \
+    *nblocks = grid;                                                                                 \
+    break;                                                                                           \
+  }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported nx " + std::to_string(nx));
+  }
+  return 0;
+}
+
+template <typename R>
 int col_pass(int n, bool inv, c2<R>* data, long W, long npanels, hipStream_t s) {
   const c2<R>* tw = twiddle_table<R>(n);
   if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
@@ -151,6 +174,8 @@ int row_pass(int n, bool inv, c2<R>* data, long nlines, hipStream_t s) {
 #define PF_INST(R)                                                                \
   template int x_r2c<R>(int, c2<R>*, c2<R>*, long, R, hipStream_t);               \
   template int x_c2r<R>(int, c2<R>*, const c2<R>*, long, hipStream_t);            \
+  template int x_c2r_sumsq<R>(int, const c2<R>*, const c2<R>*, long, long, double, double, double*, long, \
+                              long*, hipStream_t);                                 \
   template int col_pass<R>(int, bool, c2<R>*, long, long, hipStream_t);           \
   template int row_pass<R>(int, bool, c2<R>*, long, hipStream_t);
 PF_INST(float)

@@ -43,6 +43,12 @@ template <typename R>
 int x_c2r(int nx, c2<R>* data, const c2<R>* nyq, long nlines, hipStream_t s);
 template <typename R>
 int col_pass(int n, bool inv, c2<R>* data, long W, long npanels, hipStream_t s);
+// Inverse x pass of a batch of gradient fields reduced to
+// sum_lines w * sum_points out^2 (w = w0 for the lines of gradient fields 0-2,
+// w1 after), one partial per block (at most max_blocks) in part.
+template <typename R>
+int x_c2r_sumsq(int nx, const c2<R>* data, const c2<R>* nyq, long nlines, long lines_per_field, double w0,
+                double w1, double* part, long max_blocks, long* nblocks, hipStream_t s);
 template <typename R>
 int row_pass(int n, bool inv, c2<R>* data, long nlines, hipStream_t s);
 

@@ -48,7 +48,7 @@ __host__ __device__ constexpr int lines_per_block(int N) {
 template <typename R, int N, bool INV>
 __global__ void __launch_bounds__(col_tile(N, sizeof(c2<R>)) * plan_T(N))
 k_col_fft(c2<R>* __restrict__ data, long W, long tiles_per_panel, const c2<R>* __restrict__ tw) {
-  constexpr int T = plan_T(N), E = plan_E(N);
+  constexpr int E = plan_E(N);
   constexpr int TX = col_tile(N, sizeof(c2<R>));
   __shared__ c2<R> sh[(plan_np(N) > 1) ? N * TX : 1];
   const long b = blockIdx.x / tiles_per_panel;
@@ -192,6 +192,58 @@ k_x_c2r(c2<R>* __restrict__ data, const c2<R>* __restrict__ nyq, long nlines,
 #pragma unroll
   for (int s = 0; s < kOutSlots<M>; ++s)
     if (act && rfft_out_valid<M>(t, s)) g[rfft_out_idx<M>(t, s)] = v[s];
+}
+
+// ---------------------------------------------------------------------------
+// x pass, inverse, reduced to weighted sums of squares instead of stored: for
+// the gradient-energy sum of a batch of gradient fields (three per state field,
+// lines_per_field lines each), w = w0 for state field 0 and w1 for the others.
+// Block partials (fixed order) in part[blockIdx.x].
+// ---------------------------------------------------------------------------
+template <typename R, int NX>
+__global__ void __launch_bounds__(lines_per_block(NX / 2) * plan_T(NX / 2))
+k_x_c2r_sumsq(const c2<R>* __restrict__ data, const c2<R>* __restrict__ nyq, long nlines, long lines_per_field,
+              double w0, double w1, const c2<R>* __restrict__ tw, double* __restrict__ part) {
+  constexpr int M = NX / 2, T = plan_T(M), E = plan_E(M);
+  constexpr int LPB = lines_per_block(M), NT = LPB * T;
+  __shared__ c2<R> sh[(plan_np(M) > 1) ? LPB * M : 1];
+  __shared__ double red[NT / 64 + 1];
+  const int l = threadIdx.x / T, t = threadIdx.x % T;
+  double acc = 0.0;
+  const long ngroups = (nlines + LPB - 1) / LPB;
+  for (long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const long line = grp * LPB + l;
+    const bool act = line < nlines;
+    const c2<R>* g = data + (act ? line : 0) * M;
+    c2<R> v[E];
+#pragma unroll
+    for (int s = 0; s < kInSlots<M>; ++s) {
+      if (!(act && rfft_in_valid<M>(t, s))) continue;
+      const int k = rfft_in_idx<M>(t, s);
+      if (k == 0) {
+        const R a0 = g[0].x, xn = nyq[line].x;
+        v[s] = mk<R>(a0 + xn, a0 - xn);
+      } else {
+        v[s] = c2r_pre<R>(g[k], g[M - k], tw[k]);
+      }
+    }
+    rfft<R, M, true, 1>(v, t, act, sh + ((plan_np(M) > 1) ? l * M : 0), tw, 2);
+    double q = 0.0;
+#pragma unroll
+    for (int s = 0; s < kOutSlots<M>; ++s)
+      if (act && rfft_out_valid<M>(t, s)) q += (double)v[s].x * v[s].x + (double)v[s].y * v[s].y;
+    if (act) acc += ((line / lines_per_field) / 3 == 0 ? w0 : w1) * q;
+    __syncthreads();  // the next group reuses the exchange tile
+  }
+  // block reduction in a fixed order
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int w = 0; w < (NT + 63) / 64; ++w) sum += red[w];
+    part[blockIdx.x] = sum;
+  }
 }
 
 }  // namespace pf

@@ -223,35 +223,53 @@ k_nl_reduced(Geo g, UCoef u, R* __restrict__ w, R* __restrict__ mobd, double* __
 // Faithful k-space kernels (FP64).  One block per (ky, kz, field) row, threads
 // over kx in [0, M] (kx = M: Nyquist plane).  Grid-stride over rows.
 // ---------------------------------------------------------------------------
-// loops 1 and 3: dft_grad_d = dft * kf_d  (src/var_diff.f90:577-579, 645-647)
-__global__ void k_grad_spec(Geo g, KTab kt, Bank<double> st, Bank<double> gr, long nrows) {
-  for (long row = blockIdx.x; row < nrows; row += gridDim.x) {
-    const RowIdx r = row_decode(g, row);
-    for (int kx = threadIdx.x; kx <= g.M; kx += blockDim.x) {
-      const double2 v = *elem(st, g, r.f, kx, r.ky, r.kz);
-      const double a[3] = {kt.axp[kx], kt.ayp[r.ky], kt.azp[r.kz]};
-      for (int d = 0; d < 3; ++d)
-        *elem(gr, g, 3 * r.f + d, kx, r.ky, r.kz) = make_double2(-(v.y * a[d]), v.x * a[d]);
+
+#define PF_FAITHFUL_SIZES(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(160) X(192) X(256) X(512)
+
+// Loops 1 and 3 (dft_grad_d = dft * kf_d, src/var_diff.f90:577-579, 645-647)
+// fused with the first inverse pass of the gradients: for
+// every z column of state field f, the three gradient spectra i a'_d u^
+// (a' = Nyquist-zeroed wavenumbers) are formed in registers, transformed back
+// along z and stored as gradient fields 3f+d; the spectra themselves are never
+// stored.  NYQ: the Nyquist planes (columns = ky, a'_x = 0).
+template <int NZ, bool NYQ>
+__global__ void __launch_bounds__(col_tile(NZ, sizeof(double2)) * plan_T(NZ))
+k_grad_zinv(Geo g, KTab kt, const double2* __restrict__ src, double2* __restrict__ dst, long W,
+            long tiles_per_field, const double2* __restrict__ tw) {
+  constexpr int E = plan_E(NZ);
+  constexpr int TX = col_tile(NZ, sizeof(double2));
+  __shared__ double2 sh[(plan_np(NZ) > 1) ? NZ * TX : 1];
+  const long f = blockIdx.x / tiles_per_field;
+  const long col0 = (blockIdx.x - f * tiles_per_field) * TX;
+  const int ncol = (int)((W - col0) < TX ? (W - col0) : TX);
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const bool act = c < ncol;
+  const long col = col0 + (act ? c : 0);
+  const double ax = NYQ ? kt.axp[g.M] : kt.axp[col % g.M];
+  const double ay = NYQ ? kt.ayp[col] : kt.ayp[col / g.M];
+  const double2* in = src + f * NZ * W + col;
+  double2 u[E], v[E];
+#pragma unroll
+  for (int s = 0; s < kInSlots<NZ>; ++s)
+    if (act && rfft_in_valid<NZ>(t, s)) u[s] = in[(long)rfft_in_idx<NZ>(t, s) * W];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int s = 0; s < kInSlots<NZ>; ++s) {
+      if (!(act && rfft_in_valid<NZ>(t, s))) continue;
+      const double a = (d == 0) ? ax : (d == 1) ? ay : kt.azp[rfft_in_idx<NZ>(t, s)];
+      v[s] = make_double2(-(u[s].y * a), u[s].x * a);
     }
+    if (d > 0) __syncthreads();  // the exchange tile is reused
+    rfft<double, NZ, true, TX>(v, t, act, sh + c, tw);
+    double2* out = dst + (3 * f + d) * NZ * W + col;
+#pragma unroll
+    for (int s = 0; s < kOutSlots<NZ>; ++s)
+      if (act && rfft_out_valid<NZ>(t, s)) out[(long)rfft_out_idx<NZ>(t, s) * W] = v[s];
   }
 }
 
-// loops 2 and 4: f_int += 0.5*kappa*((gx^2+gy^2)+gz^2)  (src/var_diff.f90:620-622, 670-671)
-__global__ void __launch_bounds__(NB)
-k_fint_faithful(Geo g, UCoef u, const double* __restrict__ gr, double* __restrict__ part) {
-  __shared__ double red[NB];
-  double acc = 0.0;
-  const long total = (long)g.F * g.P;
-  for (long q = (long)blockIdx.x * NB + threadIdx.x; q < total; q += (long)gridDim.x * NB) {
-    const long f = q / g.P, t = q - f * g.P;
-    const double* b = gr + 3 * f * g.P + t;
-    const double gx = b[0], gy = b[g.P], gz = b[2 * g.P];
-    const double kap = (f == 0) ? u.kc : u.kphi;
-    acc += (0.5 * kap) * ((gx * gx + gy * gy) + gz * gz);
-  }
-  const double s = block_sum<NB>(acc, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
 
 // loop 11: dft_dummy = kf_sum*dft_df_dc - grad_coeff_c*dft_con*k_cube  (src/var_diff.f90:932-936)
 template <typename R>
@@ -572,6 +590,27 @@ int ensure_diag(pf_handle* h, long n) {
 // ---------------------------------------------------------------------------
 // Faithful step (FP64): the reference's loop sequence.
 // ---------------------------------------------------------------------------
+int launch_grad_zinv(pf_handle* h) {
+  const Geo& g = h->g;
+  const double2* tw = twiddle_table<double>(g.nz);
+  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  switch (g.nz) {
+#define PF_CASE(N)                                                                                      \
+  case N: {                                                                                             \
+    constexpr int TX = col_tile(N, sizeof(double2));                                                    \
+    const long Wm = (long)g.M * g.ny, tm = (Wm + TX - 1) / TX, tn = ((long)g.ny + TX - 1) / TX;         \
+    hipLaunchKernelGGL((k_grad_zinv<N, false>), dim3((unsigned)(tm * g.F)), dim3(TX * plan_T(N)), 0,    \
+                       h->s, g, h->kt, h->fs.main, h->fgrad.main, Wm, tm, tw);                           \
+    hipLaunchKernelGGL((k_grad_zinv<N, true>), dim3((unsigned)(tn * g.F)), dim3(TX * plan_T(N)), 0,     \
+                       h->s, g, h->kt, h->fs.nyq, h->fgrad.nyq, (long)g.ny, tn, tw);                     \
+    return 0;                                                                                           \
+  }
+    PF_FAITHFUL_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported nz");
+  }
+}
+
 int step_faithful(pf_handle* h, bool diag, long rec) {
   const Geo& g = h->g;
   hipStream_t s = h->s;
@@ -598,12 +637,19 @@ int step_faithful(pf_handle* h, bool diag, long rec) {
   // is the reference's either way.
   if (diag) {
     tbeg(h, ST_GRAD);
-    hipLaunchKernelGGL(k_grad_spec, dim3(grid_for((long)g.plane * g.F, 1)), dim3(rb), 0, s, g, h->kt, h->fs,
-                       h->fgrad, (long)g.plane * g.F);
-    if ((rc = inv3d<double>(h->fgrad.main, h->fgrad.nyq, g.nx, g.ny, g.nz, 3 * g.F, s))) return rc;
-    hipLaunchKernelGGL(k_fint_faithful, dim3(grid_for((long)g.F * g.P, NB)), dim3(NB), 0, s, g, h->u,
-                       (double*)h->fgrad.main, h->part_fi);
-    h->n_part_fi = grid_for((long)g.F * g.P, NB);
+    if ((rc = launch_grad_zinv(h))) return rc;
+    // loops 2 and 4 (src/var_diff.f90:620-622, 670-671): inverse y of the 3F
+    // gradient fields, then the inverse x pass summed straight into
+    // f_int = sum 0.5*kappa*|grad u|^2 (the real gradients are never stored)
+    const long M = g.M, nb3 = 3L * g.F;
+    if ((rc = col_pass<double>(g.ny, true, h->fgrad.main, M, (long)g.nz * nb3, s))) return rc;
+    if ((rc = row_pass<double>(g.ny, true, h->fgrad.nyq, (long)g.nz * nb3, s))) return rc;
+    long nblk = 0;
+    if ((rc = x_c2r_sumsq<double>(g.nx, h->fgrad.main, h->fgrad.nyq, (long)g.plane * nb3, g.plane,
+                                  0.5 * h->u.kc, 0.5 * h->u.kphi, h->part_fi,
+                                  2 * std::max(kMaxGrid, kZlineMaxGrid), &nblk, s)))
+      return rc;
+    h->n_part_fi = nblk;
     tend(h);
   }
   if (diag) {
