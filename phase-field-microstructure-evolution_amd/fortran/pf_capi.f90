@@ -34,6 +34,7 @@ module pf_capi
 
   public :: pf_create, pf_destroy, pf_put_state, pf_get_state, pf_step, pf_diagnose
   public :: pf_set_timing, pf_stage_times, pf_stage_name, pf_f_trans, pf_inv_trans
+  public :: pf_f_trans_z, pf_inv_trans_z, pf_k_space_z
   public :: pf_ran2_init, pf_ran2, pf_k_space, pf_gamma, pf_last_error, pf_analyze_state
   public :: pf_error_message, pf_check, ran_2
 
@@ -97,6 +98,24 @@ module pf_capi
       real(c_double), intent(in) :: in(*)
       real(c_double), intent(out) :: out(*)
       integer(c_int), value :: nx, ny, nz
+    end function
+    ! The same three entry points with complex spectra (C sees interleaved doubles)
+    integer(c_int) function pf_f_trans_z(in, out, nx, ny, nz) bind(C, name="pf_f_trans")
+      import :: c_int, c_double, c_double_complex
+      real(c_double), intent(in) :: in(*)
+      complex(c_double_complex), intent(out) :: out(*)
+      integer(c_int), value :: nx, ny, nz
+    end function
+    integer(c_int) function pf_inv_trans_z(in, out, nx, ny, nz) bind(C, name="pf_inv_trans")
+      import :: c_int, c_double, c_double_complex
+      complex(c_double_complex), intent(in) :: in(*)
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), value :: nx, ny, nz
+    end function
+    integer(c_int) function pf_k_space_z(nx, ny, nz, kf, kf_sq, kf_4) bind(C, name="pf_k_space")
+      import :: c_int, c_double_complex
+      integer(c_int), value :: nx, ny, nz
+      complex(c_double_complex), intent(out) :: kf(*), kf_sq(*), kf_4(*)
     end function
     subroutine pf_ran2_init(st) bind(C, name="pf_ran2_init")
       import :: pf_ran2_state
