@@ -30,4 +30,19 @@ template <typename R, bool INV> __device__ __forceinline__ c2<R> mul_mi(c2<R> a)
   return INV ? mk<R>(-a.y, a.x) : mk<R>(a.y, -a.x);
 }
 
+// Streaming global accesses: the fused passes read and write each element of
+// a bank once, so these carry the non-temporal hint (no retention in L2/MALL).
+template <typename R> __device__ __forceinline__ c2<R> ld_nt(const c2<R>* p) {
+  typedef R N __attribute__((ext_vector_type(2)));
+  const N v = __builtin_nontemporal_load(reinterpret_cast<const N*>(p));
+  return mk<R>(v.x, v.y);
+}
+template <typename R> __device__ __forceinline__ void st_nt(c2<R>* p, c2<R> a) {
+  typedef R N __attribute__((ext_vector_type(2)));
+  N v;
+  v.x = a.x;
+  v.y = a.y;
+  __builtin_nontemporal_store(v, reinterpret_cast<N*>(p));
+}
+
 }  // namespace pf

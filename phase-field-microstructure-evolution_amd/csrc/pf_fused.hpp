@@ -192,7 +192,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
           const c2<R>* src = wm + (long)f * g.hmain + (L0 + l) * M;
 #pragma unroll
           for (int kk = 0; kk < KPL; ++kk)
-            if (M % 64 == 0 || lane + 64 * kk < M) tmp[r][kk] = src[lane + 64 * kk];
+            if (M % 64 == 0 || lane + 64 * kk < M) tmp[r][kk] = ld_nt<R>(src + lane + 64 * kk);
         }
       }
 #pragma unroll
@@ -266,7 +266,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
 #pragma unroll
       for (int kk = 0; kk < KPL; ++kk) {
         const int k = lane + 64 * kk;
-        if (M % 64 == 0 || k < M) dst[k] = r2c_post<R>(rw[k], rw[(M - k) % M], shw[k], hs);
+        if (M % 64 == 0 || k < M) st_nt<R>(dst + k, r2c_post<R>(rw[k], rw[(M - k) % M], shw[k], hs));
       }
     }
     for (int i = tid; i < njobs; i += nt) {
@@ -311,11 +311,11 @@ k_ypass(c2<R>* __restrict__ main, c2<R>* __restrict__ nyq, long M, long npanels,
     c2<R>* gp = main + pnl * NY * M + col0 + c;
 #pragma unroll
     for (int s = 0; s < kInSlots<NY>; ++s)
-      if (act && rfft_in_valid<NY>(t, s)) v[s] = gp[(long)rfft_in_idx<NY>(t, s) * M];
+      if (act && rfft_in_valid<NY>(t, s)) v[s] = ld_nt<R>(gp + (long)rfft_in_idx<NY>(t, s) * M);
     rfft<R, NY, INV, TX>(v, t, act, sh + c, tw);
 #pragma unroll
     for (int s = 0; s < kOutSlots<NY>; ++s)
-      if (act && rfft_out_valid<NY>(t, s)) gp[(long)rfft_out_idx<NY>(t, s) * M] = v[s];
+      if (act && rfft_out_valid<NY>(t, s)) st_nt<R>(gp + (long)rfft_out_idx<NY>(t, s) * M, v[s]);
   } else {
     const int l = threadIdx.x / T, t = threadIdx.x % T;
     const long line = (b - nmain) * TX + l;
@@ -323,11 +323,11 @@ k_ypass(c2<R>* __restrict__ main, c2<R>* __restrict__ nyq, long M, long npanels,
     c2<R>* gp = nyq + (act ? line : 0) * NY;
 #pragma unroll
     for (int s = 0; s < kInSlots<NY>; ++s)
-      if (act && rfft_in_valid<NY>(t, s)) v[s] = gp[rfft_in_idx<NY>(t, s)];
+      if (act && rfft_in_valid<NY>(t, s)) v[s] = ld_nt<R>(gp + rfft_in_idx<NY>(t, s));
     rfft<R, NY, INV, 1>(v, t, act, sh + ((plan_np(NY) > 1) ? l * NY : 0), tw);
 #pragma unroll
     for (int s = 0; s < kOutSlots<NY>; ++s)
-      if (act && rfft_out_valid<NY>(t, s)) gp[rfft_out_idx<NY>(t, s)] = v[s];
+      if (act && rfft_out_valid<NY>(t, s)) st_nt<R>(gp + rfft_out_idx<NY>(t, s), v[s]);
   }
 }
 
@@ -396,11 +396,11 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
     // 1. all loads of the tile at once: transform input, spectrum (and f^) at the output rows
 #pragma unroll
     for (int s = 0; s < NI; ++s)
-      if (act && rfft_in_valid<NZ>(t, s)) v[s] = ib[(long)rfft_in_idx<NZ>(t, s) * Wd];
+      if (act && rfft_in_valid<NZ>(t, s)) v[s] = ld_nt<R>(ib + (long)rfft_in_idx<NZ>(t, s) * Wd);
 #pragma unroll
     for (int s = 0; s < NO; ++s)
       if (act && rfft_out_valid<NZ>(t, s)) {
-        sv[s] = sb[(long)rfft_out_idx<NZ>(t, s) * Wd];
+        sv[s] = ld_nt<R>(sb + (long)rfft_out_idx<NZ>(t, s) * Wd);
         if constexpr (ROLE == ROLE_CMOB) xv[s] = xb[(long)rfft_out_idx<NZ>(t, s) * Wd];
       }
     const long col = col0 + c;
@@ -466,7 +466,7 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
           if (f != 0) dc64[f] = dc64[f] - (u.L * u.dt) * (double)w.x;
           sn = mk<R>((R)dc64[f], R(0));
         }
-        sb[ga] = sn;
+        st_nt<R>(sb + ga, sn);
 
       }
       v[s] = sn;
@@ -503,7 +503,7 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
     c2<R>* const ob = (ROLE == ROLE_GHAT) ? (isn ? Y.nyq : Y.main) + off0 : wb;
 #pragma unroll
     for (int s = 0; s < NO; ++s)
-      if (act && rfft_out_valid<NZ>(t, s)) ob[(long)rfft_out_idx<NZ>(t, s) * Wd] = v[s];
+      if (act && rfft_out_valid<NZ>(t, s)) st_nt<R>(ob + (long)rfft_out_idx<NZ>(t, s) * Wd, v[s]);
     __syncthreads();
   }
   if (PARSEVAL) {
