@@ -60,11 +60,11 @@ k_col_fft(c2<R>* __restrict__ data, long W, long tiles_per_panel, const c2<R>* _
   c2<R> v[E];
 #pragma unroll
   for (int s = 0; s < kInSlots<N>; ++s)
-    if (act && rfft_in_valid<N>(t, s)) v[s] = g[(long)rfft_in_idx<N>(t, s) * W];
+    if (act && rfft_in_valid<N>(t, s)) v[s] = ld_nt<R>(g + (long)rfft_in_idx<N>(t, s) * W);
   rfft<R, N, INV, TX>(v, t, act, sh + c, tw);
 #pragma unroll
   for (int s = 0; s < kOutSlots<N>; ++s)
-    if (act && rfft_out_valid<N>(t, s)) g[(long)rfft_out_idx<N>(t, s) * W] = v[s];
+    if (act && rfft_out_valid<N>(t, s)) st_nt<R>(g + (long)rfft_out_idx<N>(t, s) * W, v[s]);
 }
 
 // ---------------------------------------------------------------------------
@@ -83,11 +83,11 @@ k_row_fft(c2<R>* __restrict__ data, long nlines, const c2<R>* __restrict__ tw) {
   c2<R> v[E];
 #pragma unroll
   for (int s = 0; s < kInSlots<N>; ++s)
-    if (act && rfft_in_valid<N>(t, s)) v[s] = g[rfft_in_idx<N>(t, s)];
+    if (act && rfft_in_valid<N>(t, s)) v[s] = ld_nt<R>(g + rfft_in_idx<N>(t, s));
   rfft<R, N, INV, 1>(v, t, act, sh + ((plan_np(N) > 1) ? l * N : 0), tw);
 #pragma unroll
   for (int s = 0; s < kOutSlots<N>; ++s)
-    if (act && rfft_out_valid<N>(t, s)) g[rfft_out_idx<N>(t, s)] = v[s];
+    if (act && rfft_out_valid<N>(t, s)) st_nt<R>(g + rfft_out_idx<N>(t, s), v[s]);
 }
 
 // ---------------------------------------------------------------------------
@@ -141,7 +141,7 @@ k_x_r2c(c2<R>* __restrict__ data, c2<R>* __restrict__ nyq, long nlines,
   c2<R> v[E];
 #pragma unroll
   for (int s = 0; s < kInSlots<M>; ++s)
-    if (act && rfft_in_valid<M>(t, s)) v[s] = g[(long)l * M + rfft_in_idx<M>(t, s)];
+    if (act && rfft_in_valid<M>(t, s)) v[s] = ld_nt<R>(g + (long)l * M + rfft_in_idx<M>(t, s));
   c2<R>* line = sh + l * M;
   rfft<R, M, false, 1>(v, t, act, line, tw, 2);
   __syncthreads();
@@ -153,7 +153,7 @@ k_x_r2c(c2<R>* __restrict__ data, c2<R>* __restrict__ nyq, long nlines,
   for (int i = threadIdx.x; i < nl * M; i += NT) {
     const int ll = i / M, k = i - ll * M;
     const c2<R>* z = sh + ll * M;
-    g[i] = r2c_post<R>(z[k], z[(M - k) % M], tw[k], hs);
+    st_nt<R>(g + i, r2c_post<R>(z[k], z[(M - k) % M], tw[k], hs));
   }
   if (threadIdx.x < nl) {
     const c2<R> z0 = sh[threadIdx.x * M];
@@ -191,7 +191,7 @@ k_x_c2r(c2<R>* __restrict__ data, const c2<R>* __restrict__ nyq, long nlines,
   rfft<R, M, true, 1>(v, t, act, sh + ((plan_np(M) > 1) ? l * M : 0), tw, 2);
 #pragma unroll
   for (int s = 0; s < kOutSlots<M>; ++s)
-    if (act && rfft_out_valid<M>(t, s)) g[rfft_out_idx<M>(t, s)] = v[s];
+    if (act && rfft_out_valid<M>(t, s)) st_nt<R>(g + rfft_out_idx<M>(t, s), v[s]);
 }
 
 // ---------------------------------------------------------------------------
