@@ -71,10 +71,19 @@ __device__ __forceinline__ long xcd_block(long b, long nb) {
 
 // Nonlinear terms for one point, reduced O(var) form (gamma block structure of
 // src/var_diff.f90:206-218).  v: field 0 = c, fields 1..V = eta, at stride fs.
+// Per matrix field the derivative
+//   eta_i (a_i - 1) + 2 eta_i (s1g (sm - a_i) + s2g av) + cT4m eta_i a_i,  a_i = eta_i^2
+// is evaluated as eta_i (alpha a_i + beta) with the point constants
+//   alpha = 1 - 2 s1g + cT4m,  beta = 2 (s1g sm + s2g av) - 1,
+// one multiply-add and two multiplies per field; the precipitate field's
+// derivative likewise as eta_V (gamma av + delta).  The regrouping moves each
+// derivative by a few ulp of beta (the derivative enters the update multiplied
+// by dt L).  Extremes are tracked in the working precision (exact: they are
+// values of the field); the two energy sums in FP64.
 template <typename R, bool DIAG, bool WRITE>
 __device__ __forceinline__ void nl_point(R* v, long fs, int V, const UPar& u, R& mob_out,
-                                         double& a_ch, double& a_eta, double& cmax, double& cmin,
-                                         double& pmin, double& p3) {
+                                         double& a_ch, double& a_eta, R& cmax, R& cmin,
+                                         R& pmin, R& p3) {
   const R am = (R)u.am, ap = (R)u.ap, cm = (R)u.cm, cp = (R)u.cp, s1g = (R)u.s1g, s2g = (R)u.s2g;
   const int mat = V - 1;
   const R con = v[0];
@@ -83,28 +92,29 @@ __device__ __forceinline__ void nl_point(R* v, long fs, int V, const UPar& u, R&
   for (int i = 1; i <= mat; ++i) {
     const R fi = v[i * fs];
     const R a = fi * fi;
-    pair += a * sm;
+    if (DIAG) pair += a * sm;
     sm += a;
     num += a * a;
     if (DIAG) fe += a * (R(0.25) * a - R(0.5));
   }
   const R fv = v[V * fs], av = fv * fv, av2 = av * av;
   const R d4 = num + av2;
-  const R hm = num / d4, hp = R(1) - hm;
+  const R inv_d4 = R(1) / d4;
+  const R hm = num * inv_d4, hp = R(1) - hm;
   const R dm = con - cm, dp = con - cp;
   const R fm = am * (dm * dm), fpp = ap * (dp * dp);
-  const R inv_d4 = R(1) / d4;
   if (WRITE) {
-    v[0] = R(2) * hm * am * dm + R(2) * hp * ap * dp;
+    v[0] = R(2) * (hm * am * dm + hp * ap * dp);
     const R cT4m = R(4) * (fm - fpp) * (av2 * inv_d4) * inv_d4;
-#pragma unroll 2
+    const R alpha = (R(1) - R(2) * s1g) + cT4m;
+    const R beta = R(2) * (s1g * sm + s2g * av) - R(1);
+#pragma unroll 4
     for (int i = 1; i <= mat; ++i) {
-      const R fi = v[i * fs], a = fi * fi;
-      const R ex = s1g * (sm - a) + s2g * av;
-      v[i * fs] = fi * (a - R(1)) + R(2) * fi * ex + cT4m * fi * a;
+      const R fi = v[i * fs];
+      v[i * fs] = fi * (fi * fi * alpha + beta);
     }
-    const R T4v = R(4) * fv * av * hm * (fpp - fm) * inv_d4;
-    v[V * fs] = fv * (av - R(1)) + R(2) * fv * s2g * sm + T4v;
+    const R gam = R(1) + R(4) * hm * (fpp - fm) * inv_d4;
+    v[V * fs] = fv * (av * gam + (R(2) * s2g * sm - R(1)));
   }
   const R ps = sm + av;
   mob_out = R(2) * (R(1) - ps) * (R)u.Dgb;
@@ -112,11 +122,10 @@ __device__ __forceinline__ void nl_point(R* v, long fs, int V, const UPar& u, R&
     fe += av * (R(0.25) * av - R(0.5)) + s1g * pair + s2g * av * sm;
     a_ch += (double)(hm * fm + hp * fpp);
     a_eta += (double)fe;
-    const double dfv = fv, dc = con, dps = ps;
-    if (dfv > p3) p3 = dfv;
-    if (dc <= cmin) cmin = dc;
-    if (dc > cmax) cmax = dc;
-    if (dps < pmin) pmin = dps;
+    if (fv > p3) p3 = fv;
+    if (con <= cmin) cmin = con;
+    if (con > cmax) cmax = con;
+    if (ps < pmin) pmin = ps;
   }
 }
 
@@ -172,7 +181,8 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const int row = act ? job : njobs;
   c2<R>* col = sh + row * MP;
   const R scale = (R)u.invP;
-  double a_ch = 0.0, a_eta = 0.0, cmax = 0.0, cmin = 1.0, pmin = 1.0, p3 = 0.0;
+  double a_ch = 0.0, a_eta = 0.0;
+  R cmax = 0, cmin = 1, pmin = 1, p3 = 0;
   c2<R> v[E];
 
   const long lb = xcd_block(blockIdx.x, gridDim.x);
@@ -278,8 +288,8 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   }
   if (!XONLY && diag) {
     const double s0 = bsum(a_ch, red), s1 = bsum(a_eta, red);
-    const double e0 = bext<true>(cmax, red), e1 = bext<false>(cmin, red);
-    const double e2 = bext<false>(pmin, red), e3 = bext<true>(p3, red);
+    const double e0 = bext<true>((double)cmax, red), e1 = bext<false>((double)cmin, red);
+    const double e2 = bext<false>((double)pmin, red), e3 = bext<true>((double)p3, red);
     if (tid == 0) {
       double* o = part + (long)blockIdx.x * 6;
       o[0] = s0; o[1] = s1; o[2] = e0; o[3] = e1; o[4] = e2; o[5] = e3;
