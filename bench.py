@@ -41,16 +41,25 @@ def workload_params(pf):
                           con_0_mat=0.1, con_0_ppt=1.0)
 
 
-def run_mode(pf, torch, mode, n, var, phi, con, steps, warmup, timing=True):
+def run_mode(pf, torch, mode, n, var, phi, con, steps, warmup, timing=True, graphs=True, stages_after=False):
+    """Times `steps` steps (summary rows every 2nd step) after `warmup`.  With
+    timing, per-stage HIP events are recorded inside the timed region (and the
+    step is launched kernel by kernel); with stages_after, the timed region runs
+    without events (graph replay when graphs) and the stage times come from a
+    second run of the same length afterwards."""
     with pf.Simulation(workload_params(pf), n, n, n, var, mode) as sim:
+        sim.set_graphs(graphs)
         sim.put_state(phi, con)
         sim.step(warmup, diag_every=2)
-        sim.set_timing(timing)
+        sim.set_timing(timing and not stages_after)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         sim.step(steps, diag_every=2)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        if timing and stages_after:
+            sim.set_timing(True)
+            sim.step(steps, diag_every=2)
         stages = sim.stage_times()
     return dt, stages
 
@@ -172,6 +181,8 @@ def ensemble(pf, torch, args):
     pts = sum(m["grid"] ** 3 for m in members)
     log(f"ensemble: {len(members)} simulations, {pts / 1e6:.1f} M points in total")
     sims = rc.run_ensemble(pf, members, 20)
+    for s in sims:
+        s.set_graphs(not args.no_graphs)
     try:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -216,6 +227,11 @@ def main():
                     help="instead of the single grid: N independent FP32 simulations, grids cycling "
                          "64..192^3, var 8, parameters from one ran_2 stream, advanced together")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel instead of replaying captured step periods")
+    ap.add_argument("--stages-after", action="store_true",
+                    help="time the steps without per-stage events (graph replay) and take the "
+                         "stage times from a second run afterwards")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes) and exit")
@@ -258,7 +274,8 @@ def main():
         return
 
     log(f"fp32_mixed: {args.warmup} warmup + {args.steps} timed steps")
-    dt, stages = run_mode(pf, torch, "fp32_mixed", n, var, phi, con, args.steps, args.warmup)
+    dt, stages = run_mode(pf, torch, "fp32_mixed", n, var, phi, con, args.steps, args.warmup,
+                          graphs=not args.no_graphs, stages_after=args.stages_after)
     ms_step = dt / args.steps * 1e3
     gpups = n ** 3 * args.steps / dt
     name, avg_ms, achieved, kbytes = roofline_of(stages, args.steps)
@@ -273,7 +290,8 @@ def main():
     if not args.no_fp64:
         for mode in ("fp64_reduced", "fp64_faithful"):
             log(f"{mode}: 2 warmup + {args.fp64_steps} timed steps")
-            d64, _ = run_mode(pf, torch, mode, n, var, phi, con, args.fp64_steps, 2, timing=False)
+            d64, _ = run_mode(pf, torch, mode, n, var, phi, con, args.fp64_steps, 2, timing=False,
+                              graphs=not args.no_graphs)
             fp64[mode] = dict(value=n ** 3 * args.fp64_steps / d64, ms_per_step=d64 / args.fp64_steps * 1e3)
             log(f"{mode}: {fp64[mode]['ms_per_step']:.3f} ms/step")
 

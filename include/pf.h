@@ -85,6 +85,13 @@ int pf_diagnose(pf_handle* h, pf_diag* out);
  * the simulations concurrently; returns once every stream has finished.  The
  * handles must be distinct. */
 int pf_ensemble_step(pf_handle* const* hs, int n, int nsteps);
+/* Graph replay of the step (no counterpart in the reference, whose loop body is
+ * host code): when on (the default), pf_step and pf_ensemble_step capture one
+ * period of steps -- diag_every steps, or one step without diagnostics -- into
+ * a hipGraph on first use and replay it for every whole period of a call, one
+ * launch per period instead of one per kernel.  Results are bit-identical to
+ * stepping without graphs.  Off while stage timing is on. */
+int pf_set_graphs(pf_handle* h, int on);
 /* Per-stage timing with HIP events on the handle's stream: enable/disable and
  * read the accumulated milliseconds and launch counts of up to n stages. */
 int pf_set_timing(pf_handle* h, int on);

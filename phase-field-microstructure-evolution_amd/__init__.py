@@ -67,7 +67,7 @@ EXPORTED = (
     "pf_set_timing", "pf_stage_times", "pf_stage_name", "pf_stage_bytes", "pf_stream",
     "pf_fft_r2c", "pf_fft_c2r", "pf_f_trans", "pf_inv_trans", "pf_ran2_init", "pf_ran2",
     "pf_k_space", "pf_gamma", "pf_ic_random", "pf_ic_voronoi", "pf_last_error", "pf_version",
-    "pf_ensemble_step", "pf_analyze_state",
+    "pf_ensemble_step", "pf_analyze_state", "pf_set_graphs",
 )
 
 
@@ -101,6 +101,7 @@ def lib():
     L.pf_step.argtypes = [vp, ip, ip, vp]
     L.pf_diagnose.argtypes = [vp, C.POINTER(Diag)]
     L.pf_set_timing.argtypes = [vp, ip]
+    L.pf_set_graphs.argtypes = [vp, ip]
     L.pf_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_long), ip]
     L.pf_stage_name.argtypes = [ip, C.c_char_p, ip]
     L.pf_stage_bytes.argtypes = [vp, ip]
@@ -212,6 +213,10 @@ class Simulation:
 
     def set_timing(self, on: bool) -> None:
         _check(lib().pf_set_timing(self._h, int(on)))
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay captured step periods (hipGraph) or launch every kernel."""
+        _check(lib().pf_set_graphs(self._h, int(on)))
 
     def stage_times(self):
         n = 32

@@ -407,7 +407,10 @@ __device__ __forceinline__ void neum(double& s, double& c, double x) {
   if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
   s = t;
 }
-__global__ void __launch_bounds__(NB) k_diag_final(DiagSrc d, double* __restrict__ out) {
+// out: the row, or (ctr != null) the base of the rows: row *ctr is written and
+// *ctr advanced by ctr_step, so that a captured step writes each replay's row.
+__global__ void __launch_bounds__(NB) k_diag_final(DiagSrc d, double* __restrict__ out, int* __restrict__ ctr,
+                                                   int ctr_step) {
   __shared__ double red[NB];
   double sch = 0, cch = 0, set = 0, cet = 0, sfi = 0, cfi = 0;
   double cmax = 0.0, cmin = 1.0, pmin = 1.0, p3 = 0.0;
@@ -427,10 +430,19 @@ __global__ void __launch_bounds__(NB) k_diag_final(DiagSrc d, double* __restrict
   const double e0 = block_ext<NB, true>(cmax, red), e1 = block_ext<NB, false>(cmin, red);
   const double e2 = block_ext<NB, false>(pmin, red), e3 = block_ext<NB, true>(p3, red);
   if (threadIdx.x == 0) {
+    if (ctr) {
+      const int row = *ctr;
+      out += 9 * (long)row;
+      *ctr = row + ctr_step;
+    }
     out[0] = fint; out[1] = fch; out[2] = feta; out[3] = (fint + fch) + feta;
     out[4] = e0; out[5] = e1; out[6] = e2; out[7] = e3;
     out[8] = d.cmean_is_float ? (double)*d.cmean_f : (d.cmean ? *d.cmean : 0.0);
   }
+}
+
+__global__ void k_set_int(int* __restrict__ p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 }  // namespace pf
@@ -470,6 +482,16 @@ struct pf_handle {
   bool fint_ready = false;  // part_fi holds f_int of the current state
   double* ddiag = nullptr;  // 9 doubles per record
   long ddiag_cap = 0;
+
+  // Captured step period (hipGraph): gx replays gx_period steps whose pattern
+  // of diagnostics is that of diag_every = gx_key (0: none).  While capturing,
+  // diagnostic rows go through the device counter dctr.
+  bool graphs = true;
+  hipGraphExec_t gx = nullptr;
+  int gx_key = -1, gx_period = 0;
+  int* dctr = nullptr;
+  bool capturing = false;
+  int cap_step = 0;
 
   double* staging = nullptr;  // (var+1)*P doubles for state I/O
 
@@ -570,12 +592,23 @@ int diag_final(pf_handle* h, long idx, bool with_nl, bool with_fi, const double*
   d.cmean = cmean;
   d.cmean_f = cmean_f;
   d.cmean_is_float = cmean_f != nullptr;
-  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(NB), 0, h->s, d, h->ddiag + 9 * idx);
+  if (h->capturing)
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(NB), 0, h->s, d, h->ddiag, h->dctr, h->cap_step);
+  else
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(NB), 0, h->s, d, h->ddiag + 9 * idx, (int*)nullptr, 0);
   return 0;
+}
+
+void drop_graph(pf_handle* h) {
+  if (h->gx) (void)hipGraphExecDestroy(h->gx);
+  h->gx = nullptr;
+  h->gx_key = -1;
+  h->gx_period = 0;
 }
 
 int ensure_diag(pf_handle* h, long n) {
   if (n <= h->ddiag_cap) return 0;
+  drop_graph(h);  // a captured period holds the old row buffer
   if (h->ddiag) {
     (void)hipFree(h->ddiag);
     h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), (void*)h->ddiag), h->allocs.end());
@@ -1060,6 +1093,12 @@ extern "C" int pf_create(const pf_params* p, int nx, int ny, int nz, int var, in
   if ((rc = dalloc(h, (void**)&h->staging, sizeof(double) * (size_t)g.P * g.F))) return bail(rc);
   h->n_part_nl = grid_for(g.P, NB);
   if ((rc = ensure_diag(h, 64))) return bail(rc);
+  if ((rc = dalloc(h, (void**)&h->dctr, sizeof(int)))) return bail(rc);
+  // the transforms' twiddle tables are made now: a step under graph capture
+  // must not allocate
+  for (int n : {nx, ny, nz})
+    if (!twiddle_table<float>(n) || !twiddle_table<double>(n))
+      return bail(fail(PF_EHIP, "twiddle table allocation failed"));
   *out = h;
   return 0;
 }
@@ -1068,6 +1107,7 @@ extern "C" int pf_destroy(pf_handle* h) {
   if (!h) return 0;
   DeviceGuard dg(h->dev);
   if (h->s) (void)hipStreamSynchronize(h->s);
+  drop_graph(h);
   for (void* p : h->allocs) (void)hipFree(p);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   if (h->s) (void)hipStreamDestroy(h->s);
@@ -1129,6 +1169,75 @@ extern "C" int pf_get_state(pf_handle* h, double* phi, double* con) {
   return 0;
 }
 
+namespace {
+
+// One step of the handle's form.  rec: the diagnostics row (when diag).
+int step_once(pf_handle* h, bool diag, long rec, bool next_diag) {
+  if (h->mode == PF_MODE_FP64_FAITHFUL) return step_faithful(h, diag, rec);
+  if (h->mode == PF_MODE_FP64_REDUCED) return step_reduced<double>(h, diag, rec, next_diag);
+  return step_reduced<float>(h, diag, rec, next_diag);
+}
+
+// Captures one period of steps with the diagnostics pattern of diag_every = de
+// (0: none) into h->gx:
+//   de = 0 : one step, no diagnostics
+//   de = 1 : one step with diagnostics that also carries f_int into the next
+//   de >= 2: de steps, the step before last carrying f_int, the last reporting
+// The capture only records; nothing runs.  On a capture failure graphs are
+// switched off for the handle and the caller steps eagerly.
+int capture_period(pf_handle* h, int de) {
+  drop_graph(h);
+  const int P = de > 0 ? de : 1;
+  const bool ready0 = h->fint_ready;
+  h->fint_ready = (de == 1);
+  h->capturing = true;
+  h->cap_step = P;
+  int rc = 0;
+  if (hipStreamBeginCapture(h->s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    h->capturing = false;
+    h->fint_ready = ready0;
+    h->graphs = false;
+    (void)hipGetLastError();
+    return 0;
+  }
+  for (int j = 0; j < P && !rc; ++j) {
+    const bool diag = de > 0 && j == P - 1;
+    const bool next = de == 1 || (de >= 2 && j == P - 2);
+    rc = step_once(h, diag, 0, next);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->s, &graph);
+  h->capturing = false;
+  h->fint_ready = ready0;
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess || !graph || hipGraphInstantiate(&h->gx, graph, nullptr, nullptr, 0) != hipSuccess) {
+    h->gx = nullptr;
+    h->graphs = false;
+    (void)hipGetLastError();
+  } else {
+    h->gx_key = de;
+    h->gx_period = P;
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+  return 0;
+}
+
+// Fewest replays of a period for which a call captures (or reuses) a graph.
+constexpr long kMinGraphReplays = 2;
+
+}  // namespace
+
+extern "C" int pf_set_graphs(pf_handle* h, int on) {
+  if (!h) return fail(PF_EINVAL, "null handle");
+  DeviceGuard dg(h->dev);
+  if (!on) drop_graph(h);
+  h->graphs = on != 0;
+  return 0;
+}
+
 extern "C" int pf_ensemble_step(pf_handle* const* hs, int n, int nsteps) {
   if (n < 0 || nsteps < 0 || (n > 0 && !hs)) return fail(PF_EINVAL, "bad ensemble arguments");
   for (int i = 0; i < n; ++i) {
@@ -1138,14 +1247,24 @@ extern "C" int pf_ensemble_step(pf_handle* const* hs, int n, int nsteps) {
     if (hs[i]->dev != hs[0]->dev) return fail(PF_EINVAL, "ensemble handles are on different devices");
   }
   DeviceGuard dg(n > 0 ? hs[0]->dev : 0);
+  // members step without diagnostics: a captured one-step period each
+  for (int i = 0; i < n && nsteps >= kMinGraphReplays; ++i) {
+    pf_handle* h = hs[i];
+    if (h->graphs && !h->timing && h->gx_key != 0) {
+      const int rc = capture_period(h, 0);
+      if (rc) return rc;
+    }
+  }
   for (int s = 0; s < nsteps; ++s) {
     for (int i = 0; i < n; ++i) {
       pf_handle* h = hs[i];
-      int rc;
-      if (h->mode == PF_MODE_FP64_FAITHFUL) rc = step_faithful(h, false, 0);
-      else if (h->mode == PF_MODE_FP64_REDUCED) rc = step_reduced<double>(h, false, 0, false);
-      else rc = step_reduced<float>(h, false, 0, false);
-      if (rc) return rc;
+      if (h->gx && h->gx_key == 0 && !h->timing && nsteps >= kMinGraphReplays) {
+        PF_HIP(hipGraphLaunch(h->gx, h->s));
+        h->fint_ready = false;
+      } else {
+        const int rc = step_once(h, false, 0, false);
+        if (rc) return rc;
+      }
       h->step += 1;
     }
   }
@@ -1164,16 +1283,40 @@ extern "C" int pf_step(pf_handle* h, int nsteps, int diag_every, pf_diag* out) {
   int rc = 0;
   if (out && (rc = ensure_diag(h, std::max(1, nsteps)))) return rc;
   std::vector<char> had(nsteps, 0);
-  for (int i = 0; i < nsteps; ++i) {
+  const int de = (out && diag_every > 0) ? diag_every : 0;
+  const int P = de > 0 ? de : 1;
+  int i = 0;
+  while (i < nsteps) {
     const long stepno = h->step + 1;
-    const bool diag = out && diag_every > 0 && (stepno % diag_every) == 0;
-    const bool next_diag = out && diag_every > 0 && ((stepno + 1) % diag_every) == 0 && (i + 1) < nsteps;
-    if (h->mode == PF_MODE_FP64_FAITHFUL) rc = step_faithful(h, diag, i);
-    else if (h->mode == PF_MODE_FP64_REDUCED) rc = step_reduced<double>(h, diag, i, next_diag);
-    else rc = step_reduced<float>(h, diag, i, next_diag);
-    if (rc) return rc;
+    // replays of whole periods: the steps from i on follow the captured pattern
+    // (the reduced forms carry f_int from the step before a reported one; the
+    // faithful form recomputes it, so every step of de = 1 is alike)
+    const bool carry = h->mode != PF_MODE_FP64_FAITHFUL;
+    long K = 0;
+    if (h->graphs && !h->timing) {
+      if (de == 0 || (de == 1 && !carry)) K = nsteps - i;
+      else if (de == 1) K = (i >= 1 && h->fint_ready) ? (nsteps - 1) - i : 0;
+      else if (stepno % de == 1 && !h->fint_ready) K = (nsteps - i) / de;
+    }
+    if (K >= kMinGraphReplays) {
+      if (h->gx_key != de && (rc = capture_period(h, de))) return rc;
+    }
+    if (K >= kMinGraphReplays && h->gx && h->gx_key == de) {
+      if (de > 0) hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, h->s, h->dctr, i + P - 1);
+      for (long k = 0; k < K; ++k) PF_HIP(hipGraphLaunch(h->gx, h->s));
+      for (long k = 0; k < K; ++k)
+        if (de > 0) had[i + k * P + P - 1] = 1;
+      h->fint_ready = carry && de == 1;
+      h->step += K * P;
+      i += (int)(K * P);
+      continue;
+    }
+    const bool diag = de > 0 && (stepno % de) == 0;
+    const bool next_diag = de > 0 && ((stepno + 1) % de) == 0 && (i + 1) < nsteps;
+    if ((rc = step_once(h, diag, i, next_diag))) return rc;
     had[i] = diag;
     h->step = stepno;
+    ++i;
   }
   PF_HIP(hipGetLastError());
   std::vector<double> hd;
