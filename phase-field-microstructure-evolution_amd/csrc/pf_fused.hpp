@@ -149,13 +149,19 @@ __device__ __forceinline__ void nl_point(R* v, long fs, int V, const UPar& u, R&
 // 128 VGPRs), which a 512^3 grid with 37 fields needs (37 transforms x 16
 // threads); FP64 blocks stop at 512.
 template <typename R> constexpr int kXlineMaxThreads = sizeof(R) == 4 ? 1024 : 512;
+// Minimum waves per SIMD asked of the register allocator.  FP32: the 1024-thread
+// bound already means 4 (128 VGPRs).  FP64: uncapped the kernel takes 202 VGPRs
+// (2 waves/SIMD, two 3-wave blocks per CU); capped at 3 waves it spills 120
+// bytes per lane and still runs faster, 1.52 -> 1.32 ms at 256^3 x 17 (4 waves
+// spill 292 bytes: 1.91 ms).
+template <typename R> constexpr int kXlineWaves = sizeof(R) == 4 ? 4 : 3;
 // Rows each wave keeps in flight when staging a line group.
 constexpr int kXlineRows = 2;
 
 __host__ __device__ constexpr int xline_row_pitch(int M) { return M | 1; }
 
 template <typename R, int NX, int LINES, bool XONLY>
-__global__ void __launch_bounds__(kXlineMaxThreads<R>)
+__global__ void __launch_bounds__(kXlineMaxThreads<R>) __attribute__((amdgpu_waves_per_eu(kXlineWaves<R>)))
 k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __restrict__ mobd,
         const c2<R>* __restrict__ tw, double* __restrict__ part, long ngroups, int diag, int mob) {
   constexpr int M = NX / 2, T = plan_T(M), E = plan_E(M);
