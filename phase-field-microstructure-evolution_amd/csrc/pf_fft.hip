@@ -4,8 +4,10 @@
 // on host buffers with the reference's f_trans / inv_trans semantics).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/pf.h"
@@ -27,6 +29,105 @@ bool grid_size_supported(int n) {
   }
 }
 const char* supported_sizes_text() { return "8 12 16 24 32 48 64 96 128 160 192 256 512"; }
+
+// ---------------------------------------------------------------------------
+// Host transfers through pinned bounce buffers.
+// ---------------------------------------------------------------------------
+
+namespace {
+constexpr size_t kBounceBytes = size_t(64) << 20;
+constexpr size_t kBounceMin = size_t(128) << 20;
+std::mutex g_bounce_mu;
+void* g_bounce[2] = {nullptr, nullptr};
+
+// memcpy split over up to 8 host threads
+void par_copy(void* dst, const void* src, size_t n) {
+  const int nth = (int)std::max(1u, std::min<unsigned>(8, std::thread::hardware_concurrency()));
+  if (nth == 1 || n < (size_t(4) << 20)) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t per = (n + nth - 1) / nth;
+  std::vector<std::thread> th;
+  for (int w = 0; w < nth; ++w) {
+    const size_t a = std::min(n, per * w), b = std::min(n, a + per);
+    if (a < b) th.emplace_back([=] { std::memcpy((char*)dst + a, (const char*)src + a, b - a); });
+  }
+  for (auto& t : th) t.join();
+}
+
+// Under g_bounce_mu.  Pinned memory is allocated once and kept until exit.
+bool bounce_ready() {
+  for (int i = 0; i < 2; ++i)
+    if (!g_bounce[i] && hipHostMalloc(&g_bounce[i], kBounceBytes, hipHostMallocDefault) != hipSuccess) {
+      g_bounce[i] = nullptr;
+      (void)hipGetLastError();
+      return false;
+    }
+  return true;
+}
+
+struct EventPair {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  bool ok = true;
+  EventPair() {
+    for (auto& x : e)
+      if (hipEventCreateWithFlags(&x, hipEventDisableTiming) != hipSuccess) ok = false;
+  }
+  ~EventPair() {
+    for (auto& x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+}  // namespace
+
+int to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_bounce_mu);
+  EventPair ev;
+  if (bytes < kBounceMin || !ev.ok || !bounce_ready()) {
+    PF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    PF_HIP(hipStreamSynchronize(s));
+    return 0;
+  }
+  const size_t nch = (bytes + kBounceBytes - 1) / kBounceBytes;
+  // chunk k goes into buffer k & 1; chunk k - 2 was moved out of it on the host
+  // in the previous iteration
+  for (size_t k = 0; k <= nch; ++k) {
+    if (k < nch) {
+      const size_t off = k * kBounceBytes, len = std::min(kBounceBytes, bytes - off);
+      PF_HIP(hipMemcpyAsync(g_bounce[k & 1], (const char*)src + off, len, hipMemcpyDeviceToHost, s));
+      PF_HIP(hipEventRecord(ev.e[k & 1], s));
+    }
+    if (k > 0) {
+      const size_t j = k - 1, off = j * kBounceBytes, len = std::min(kBounceBytes, bytes - off);
+      PF_HIP(hipEventSynchronize(ev.e[j & 1]));
+      par_copy((char*)dst + off, g_bounce[j & 1], len);
+    }
+  }
+  return 0;
+}
+
+int to_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_bounce_mu);
+  EventPair ev;
+  if (bytes < kBounceMin || !ev.ok || !bounce_ready()) {
+    PF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    PF_HIP(hipStreamSynchronize(s));
+    return 0;
+  }
+  const size_t nch = (bytes + kBounceBytes - 1) / kBounceBytes;
+  for (size_t k = 0; k < nch; ++k) {
+    const size_t off = k * kBounceBytes, len = std::min(kBounceBytes, bytes - off);
+    // buffer k & 1 is free once the copy of chunk k - 2 out of it has finished
+    if (k >= 2) PF_HIP(hipEventSynchronize(ev.e[k & 1]));
+    par_copy(g_bounce[k & 1], (const char*)src + off, len);
+    PF_HIP(hipMemcpyAsync((char*)dst + off, g_bounce[k & 1], len, hipMemcpyHostToDevice, s));
+    PF_HIP(hipEventRecord(ev.e[k & 1], s));
+  }
+  // the bounce buffers are shared: they must be drained before the lock goes
+  PF_HIP(hipStreamSynchronize(s));
+  return 0;
+}
 
 // ---------------------------------------------------------------------------
 // Twiddle tables.
@@ -374,11 +475,10 @@ extern "C" int pf_f_trans(const double* in, double* out, int nx, int ny, int nz)
     (void)hipFree(din);
     return fail(PF_ENOMEM, "device allocation failed");
   }
-  rc = hipMemcpy(din, in, sizeof(double) * P, hipMemcpyHostToDevice) == hipSuccess ? 0 : -3;
+  rc = to_device(din, in, sizeof(double) * P, nullptr);
   const double scale = 1.0 / (double)(float)((long)nx * ny * nz);
   if (!rc) rc = pf_fft_r2c(nx, ny, nz, 1, PF_PREC_FP64, din, dout, PF_LAYOUT_ZFAST, scale, nullptr);
-  if (!rc && hipMemcpy(out, dout, sizeof(double) * 2 * H, hipMemcpyDeviceToHost) != hipSuccess)
-    rc = fail(PF_EHIP, "copy back failed");
+  if (!rc) rc = to_host(out, dout, sizeof(double) * 2 * H, nullptr);
   (void)hipFree(din);
   (void)hipFree(dout);
   return rc;
@@ -395,10 +495,9 @@ extern "C" int pf_inv_trans(const double* in, double* out, int nx, int ny, int n
     (void)hipFree(din);
     return fail(PF_ENOMEM, "device allocation failed");
   }
-  rc = hipMemcpy(din, in, sizeof(double) * 2 * H, hipMemcpyHostToDevice) == hipSuccess ? 0 : -3;
+  rc = to_device(din, in, sizeof(double) * 2 * H, nullptr);
   if (!rc) rc = pf_fft_c2r(nx, ny, nz, 1, PF_PREC_FP64, din, dout, PF_LAYOUT_ZFAST, nullptr);
-  if (!rc && hipMemcpy(out, dout, sizeof(double) * P, hipMemcpyDeviceToHost) != hipSuccess)
-    rc = fail(PF_EHIP, "copy back failed");
+  if (!rc) rc = to_host(out, dout, sizeof(double) * P, nullptr);
   (void)hipFree(din);
   (void)hipFree(dout);
   return rc;

@@ -52,6 +52,14 @@ int x_c2r_sumsq(int nx, const c2<R>* data, const c2<R>* nyq, long nlines, long l
 template <typename R>
 int row_pass(int n, bool inv, c2<R>* data, long nlines, hipStream_t s);
 
+// Copies between device memory and the caller's (pageable) host arrays on
+// stream s; both return once the copy has finished.  Copies of 128 MiB and more
+// go through two pinned bounce buffers of 64 MiB (process-wide, made on first
+// use): the GPU moves one chunk while host threads move the neighbouring chunk
+// between the other buffer and the caller's array.
+int to_host(void* dst, const void* src, size_t bytes, hipStream_t s);
+int to_device(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 // Dense half-spectrum layouts at the C-ABI.
 enum { LAYOUT_XFAST = 0, LAYOUT_ZFAST = 1 };
 template <typename R>

@@ -1026,7 +1026,8 @@ template <typename R> int put_reduced(pf_handle* h, const double* phi, const dou
   // arrays in kMeanChunks fixed chunks (threads take whole chunks), chunk
   // partials combined in chunk order, so the result does not depend on the
   // thread count
-  std::vector<double> dc = host_means(phi, con, g.P, g.var);  PF_HIP(hipMemcpyAsync(h->dc64, dc.data(), sizeof(double) * g.F, hipMemcpyHostToDevice, h->s));
+  std::vector<double> dc = host_means(phi, con, g.P, g.var);
+  PF_HIP(hipMemcpyAsync(h->dc64, dc.data(), sizeof(double) * g.F, hipMemcpyHostToDevice, h->s));
   PF_HIP(hipStreamSynchronize(h->s));
   return 0;
 }
@@ -1138,9 +1139,9 @@ extern "C" int pf_put_state(pf_handle* h, const double* phi, const double* con) 
   if (!h || !phi || !con) return fail(PF_EINVAL, "null argument");
   DeviceGuard dg(h->dev);
   const Geo& g = h->g;
-  PF_HIP(hipMemcpyAsync(h->staging, phi, sizeof(double) * (size_t)g.P * g.var, hipMemcpyHostToDevice, h->s));
-  PF_HIP(hipMemcpyAsync(h->staging + (size_t)g.P * g.var, con, sizeof(double) * (size_t)g.P,
-                        hipMemcpyHostToDevice, h->s));
+  int rc = to_device(h->staging, phi, sizeof(double) * (size_t)g.P * g.var, h->s);
+  if (!rc) rc = to_device(h->staging + (size_t)g.P * g.var, con, sizeof(double) * (size_t)g.P, h->s);
+  if (rc) return rc;
   h->fint_ready = false;
   if (h->mode == PF_MODE_FP64_FAITHFUL) {
     hipLaunchKernelGGL((k_scatter_state<double>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s, h->staging,
@@ -1182,10 +1183,9 @@ extern "C" int pf_get_state(pf_handle* h, double* phi, double* con) {
                        (const float*)h->X_main, sphi, scon, g.P, g.var);
   }
   PF_HIP(hipGetLastError());
-  PF_HIP(hipMemcpyAsync(phi, sphi, sizeof(double) * (size_t)g.P * g.var, hipMemcpyDeviceToHost, h->s));
-  PF_HIP(hipMemcpyAsync(con, scon, sizeof(double) * (size_t)g.P, hipMemcpyDeviceToHost, h->s));
-  PF_HIP(hipStreamSynchronize(h->s));
-  return 0;
+  rc = to_host(phi, sphi, sizeof(double) * (size_t)g.P * g.var, h->s);
+  if (!rc) rc = to_host(con, scon, sizeof(double) * (size_t)g.P, h->s);
+  return rc;
 }
 
 namespace {
