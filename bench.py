@@ -64,6 +64,25 @@ def run_mode(pf, torch, mode, n, var, phi, con, steps, warmup, timing=True, grap
     return dt, stages
 
 
+def stream_reference(torch, nbytes):
+    """Read + write rate of a plain in-place stream over a buffer of nbytes (torch's
+    elementwise kernel, x *= 1), timed with events on torch's stream: what this
+    chip sustains for the access mix of a bank pass, beside the 8 TB/s peak."""
+    x = torch.ones(int(nbytes // 4), dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        x.mul_(1.0)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(10):
+        x.mul_(1.0)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / 10
+    del x
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
 def roofline_of(stages, steps):
     best = max(stages.items(), key=lambda kv: kv[1]["ms"])
     name, st = best
@@ -286,6 +305,11 @@ def main():
                              if v["count"] and v["bytes"] else None)
                      for k, v in stages.items() if v["count"]}
 
+    # the dominant kernel reads and writes kbytes / 2 bytes: stream the same amount
+    stream = stream_reference(torch, kbytes / 2) if kbytes else None
+    if stream:
+        log(f"stream reference (in-place x *= 1 over {kbytes / 2e9:.2f} GB): {stream:.0f} GB/s")
+
     fp64 = {}
     if not args.no_fp64:
         for mode in ("fp64_reduced", "fp64_faithful"):
@@ -328,7 +352,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name, n, var),
                      "stage": name, "kernel": " / ".join(STAGE_KERNELS.get(name, [name])),
-                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": kbytes},
+                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": kbytes,
+                     "stream_reference": {"achieved": stream, "frac": achieved / stream if stream else None,
+                                          "kernel": "torch in-place multiply over the same bytes "
+                                                    "(read + write), same run"}},
         "step_model": {"model_bytes_per_step": model_bytes,
                        "model_GBs": model_bytes / (ms_step * 1e-3) / 1e9,
                        "model_frac": model_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},

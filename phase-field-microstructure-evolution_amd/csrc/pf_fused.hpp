@@ -61,14 +61,6 @@ template <bool MAX> __device__ __forceinline__ double bext(double v, double* red
   return s;
 }
 
-// Map physical block ids to logical ones so that each XCD (blocks are dealt to
-// the 8 XCDs round-robin) works on a contiguous range: neighbouring lines then
-// share that XCD's L2 for the Nyquist-plane entries.
-__device__ __forceinline__ long xcd_block(long b, long nb) {
-  if (nb % 8) return b;
-  return (b % 8) * (nb / 8) + b / 8;
-}
-
 // Nonlinear terms for one point, reduced O(var) form (gamma block structure of
 // src/var_diff.f90:206-218).  v: field 0 = c, fields 1..V = eta, at stride fs.
 // Per matrix field the derivative
@@ -316,7 +308,7 @@ k_ypass(c2<R>* __restrict__ main, c2<R>* __restrict__ nyq, long M, long npanels,
   __shared__ c2<R> sh[(plan_np(NY) > 1) ? NY * TX : 1];
   const long tpp = (M + TX - 1) / TX;
   const long nmain = tpp * npanels;
-  const long b = blockIdx.x;
+  const long b = xcd_block(blockIdx.x, gridDim.x);
   c2<R> v[E];
   if (b < nmain) {
     const long pnl = b / tpp;
