@@ -387,7 +387,8 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
   c2<R> v[E];
   c2<R> sv[NO];
   c2<R> xv[(ROLE == ROLE_CMOB) ? NO : 1];
-  for (long tile = blockIdx.x; tile < total; tile += gridDim.x) {
+  const long tile0 = xcd_runs(blockIdx.x, gridDim.x, ((long)g.M + TX - 1) / TX);
+  for (long tile = tile0; tile < total; tile += gridDim.x) {
     const int f = f0 + (int)(tile / tpf);
     const long tt = tile % tpf;
     const bool isn = tt >= tm;

@@ -45,11 +45,22 @@ __host__ __device__ constexpr int lines_per_block(int N) {
 // line kernel's neighbouring lines then share that XCD's L2 for the Nyquist
 // entries, and a column pass's neighbouring tiles (the 128-byte pieces of the
 // same rows) are read by one XCD at about the same time: measured at 256^3 x 17,
-// column passes 3-7% faster.  The z-line kernel keeps the plain order, which
-// measured 1.05 against 1.34 ms with the remap.
+// column passes 3-7% faster.  (The z-line kernel measured 1.05 against 1.34 ms
+// with this remap; it uses xcd_runs below.)
 __device__ __forceinline__ long xcd_block(long b, long nb) {
   if (nb % 8) return b;
   return (b % 8) * (nb / 8) + b / 8;
+}
+
+// The same in runs of G: XCD x takes the blocks x*G .. x*G + G - 1 of every 8*G
+// (unchanged when the grid is not a multiple of 8*G).  The z-line kernel uses
+// runs of one main-block row (M / TX tiles), so each XCD reads whole 1 KiB rows
+// of a z plane: measured at 256^3 x 17, 1.055 -> 1.003 ms per launch (runs of
+// 8-32 tiles alike; 256 tiles 1.036, one contiguous range per XCD 1.34).
+__device__ __forceinline__ long xcd_runs(long b, long nb, long G) {
+  if (G < 1 || nb % (8 * G)) return b;
+  const long x = b % 8, i = b / 8;
+  return (i / G) * (8 * G) + x * G + i % G;
 }
 
 // ---------------------------------------------------------------------------
