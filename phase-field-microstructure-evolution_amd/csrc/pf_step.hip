@@ -830,7 +830,9 @@ int launch_ypass_fused(pf_handle* h, c2<R>* main, c2<R>* nyq, int nf) {
     constexpr int TX = col_tile(N, sizeof(c2<R>));                                              \
     const long npanels = (long)g.nz * nf, nlines = (long)g.nz * nf;                             \
     const long tpp = ((long)g.M + TX - 1) / TX;                                                 \
-    const long grid = tpp * npanels + (nlines + TX - 1) / TX;                                   \
+    /* a multiple of 8 blocks for the XCD order (xcd_block); the extra blocks fall on   */       \
+    /* Nyquist lines past the last one and touch no memory                              */       \
+    const long grid = (tpp * npanels + (nlines + TX - 1) / TX + 7) / 8 * 8;                     \
     const c2<R>* tw = twiddle_table<R>(N);                                                      \
     if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");                           \
     hipLaunchKernelGGL((k_ypass<R, N, INV>), dim3((unsigned)grid), dim3(TX * plan_T(N)), 0, \
@@ -870,7 +872,10 @@ int launch_zline(pf_handle* h, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y, doubl
   case N: {                                                                                          \
     constexpr int TX = col_tile(N, sizeof(c2<R>));                                                   \
     const long tm = ((long)g.M * g.ny + TX - 1) / TX, tn = ((long)g.ny + TX - 1) / TX;               \
-    const long grid = std::min<long>((tm + tn) * nf, kZlineMaxGrid);                                \
+    long grid = std::min<long>((tm + tn) * nf, kZlineMaxGrid);                                      \
+    /* whole XCD runs of one main-block row (xcd_runs): extra blocks find no tile */                 \
+    const long run = 8 * (((long)g.M + TX - 1) / TX);                                                \
+    if ((grid + run - 1) / run * run <= kZlineMaxGrid) grid = (grid + run - 1) / run * run;         \
     const c2<R>* tw = twiddle_table<R>(N);                                                           \
     if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");                                \
     hipLaunchKernelGGL((k_zline<R, N, ROLE, DC64, PARSEVAL>), dim3((unsigned)grid),                  \
