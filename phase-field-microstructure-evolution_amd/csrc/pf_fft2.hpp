@@ -397,6 +397,10 @@ template <int N> __host__ __device__ __forceinline__ int rfft_in_idx(int t, int 
 template <int N> __host__ __device__ __forceinline__ bool rfft_in_valid(int t, int s) {
   return PassC<N, 0>::valid(t, s / PassC<N, 0>::r);
 }
+// In the last pass LS = N / r = NB, so j < NB gives k = j and out_idx(t, q, m) =
+// j + m*NB = in_idx(t, q, m): each thread's outputs land on the elements it read
+// as that pass's inputs, and callers may write them back without a barrier.
+template <int N> constexpr bool last_pass_in_place() { return LastPass<N>::LS == LastPass<N>::NB; }
 template <int N> __host__ __device__ __forceinline__ bool rfft_out_valid(int t, int s) {
   return LastPass<N>::valid(t, s / LastPass<N>::r);
 }

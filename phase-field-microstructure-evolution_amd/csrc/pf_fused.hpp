@@ -158,6 +158,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
         const c2<R>* __restrict__ tw, double* __restrict__ part, long ngroups, int diag, int mob) {
   constexpr int M = NX / 2, T = plan_T(M), E = plan_E(M);
   constexpr int MP = xline_row_pitch(M);
+  static_assert(last_pass_in_place<M>(), "the write-backs after the x transforms rely on it");
   extern __shared__ unsigned char smem[];
   const int F = XONLY ? 1 : g.F;
   const int njobs = F * LINES;
@@ -228,9 +229,12 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       const R a0 = col[0].x, xn = shn[row];
       v[s] = (k == 0) ? mk<R>(a0 + xn, a0 - xn) : pk;
     }
-    // 3. inverse x transform; real pairs z[n] = (x[2n], x[2n+1]) back into the row
+    // 3. inverse x transform; real pairs z[n] = (x[2n], x[2n+1]) back into the row.
+    //    No barrier before the write-back: the last Stockham pass leaves each
+    //    thread's outputs at exactly the elements it loaded as that pass's inputs
+    //    (out_idx = in_idx when j < NB = LS), so no thread overwrites an element
+    //    another thread has yet to read.
     rfft<R, M, true, 1>(v, t, true, col, tw, 2);
-    __syncthreads();
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
       if (rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
@@ -259,8 +263,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
 #pragma unroll
     for (int s = 0; s < kInSlots<M>; ++s)
       if (rfft_in_valid<M>(t, s)) v[s] = col[rfft_in_idx<M>(t, s)];
-    rfft<R, M, false, 1>(v, t, true, col, tw, 2);
-    __syncthreads();
+    rfft<R, M, false, 1>(v, t, true, col, tw, 2);  // write-back without a barrier, as in 3.
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
       if (rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];

@@ -70,8 +70,23 @@ template <int N, bool INV> double check() {
 }
 
 
+// 1 when, for every thread and valid slot of the last pass, the output element
+// is the element the thread loaded as that pass's input (what lets callers
+// write the transform back into its exchange buffer without a barrier).
+template <int N> int last_pass_in_place_emul() {
+  using C = LastPass<N>;
+  for (int t = 0; t < C::T; ++t)
+    for (int q = 0; q < C::Q; ++q) {
+      if (!C::valid(t, q)) continue;
+      for (int m = 0; m < C::r; ++m)
+        if (C::out_idx(t, q, m) != C::in_idx(t, q, m)) return 0;
+    }
+  return last_pass_in_place<N>() ? 1 : 0;
+}
+
 #define RUN(N) \
-  std::printf("%d %d %.3e %.3e\n", N, plan_np(N), check<N, false>(), check<N, true>());
+  std::printf("%d %d %.3e %.3e %d\n", N, plan_np(N), check<N, false>(), check<N, true>(), \
+              last_pass_in_place_emul<N>());
 int main() {
   RUN(2) RUN(3) RUN(4) RUN(5) RUN(6) RUN(8) RUN(10) RUN(12) RUN(16) RUN(20) RUN(24) RUN(32) RUN(40)
   RUN(48) RUN(64) RUN(80) RUN(96) RUN(128) RUN(160) RUN(192) RUN(256) RUN(512)
