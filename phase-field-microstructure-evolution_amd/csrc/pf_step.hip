@@ -62,9 +62,17 @@ __global__ void k_gather_state(const R* __restrict__ src, double* __restrict__ p
 // Loops 5-10, faithful form (src/var_diff.f90:685-905), FP64, O(var^2).
 // Reads the real state bank (c, eta_i); writes df_dc, df_dphi_i, mob_c.
 // ---------------------------------------------------------------------------
+// gamma_ij of src/var_diff.f90:206-218 (the matrix pf_gamma builds): sigma_1
+// between two distinct matrix grains, sigma_2 between a grain and the
+// precipitate field eta_var (index mat), 0 on the diagonal.  Selected from (i, j)
+// instead of loaded: the same double values, without 2 V^2 loads per point.
+__device__ __forceinline__ double gam_ij(const UCoef& u, int mat, int i, int j) {
+  return (i == j) ? 0.0 : ((i == mat || j == mat) ? u.s2g : u.s1g);
+}
+
 template <bool DIAG>
 __global__ void __launch_bounds__(NB)
-k_nl_faithful(Geo g, UCoef u, const double* __restrict__ gam, const double* __restrict__ st,
+k_nl_faithful(Geo g, UCoef u, const double* __restrict__ st,
               double* __restrict__ dfdc, double* __restrict__ dfdphi, double* __restrict__ mob,
               double* __restrict__ part) {
   extern __shared__ double sh[];  // var * NB values, then NB reduction scratch
@@ -87,7 +95,7 @@ k_nl_faithful(Geo g, UCoef u, const double* __restrict__ gam, const double* __re
       fe = (fe - 0.5 * fi2) + 0.25 * (fi2 * fi2);
       for (int j = i + 1; j < V; ++j) {
         const double fj = ph[j * NB];
-        fe = fe + ((gam[i + (long)V * j] * fi2) * (fj * fj));
+        fe = fe + ((gam_ij(u, mat, i, j) * fi2) * (fj * fj));
       }
     }
     // loop 6
@@ -114,7 +122,7 @@ k_nl_faithful(Geo g, UCoef u, const double* __restrict__ gam, const double* __re
       for (int j = 0; j < V; ++j) {
         if (j == i) continue;
         const double fj = ph[j * NB];
-        term2 = term2 + ((2.0 * gam[i + (long)V * j]) * fi) * (fj * fj);
+        term2 = term2 + ((2.0 * gam_ij(u, mat, i, j)) * fi) * (fj * fj);
       }
       const double term4 = (i < mat) ? (((4.0 * ((fi * fi) * fi)) * fv4) * (fm - fp)) / den2
                                      : (t2v * (fp - fm)) / den2;
@@ -465,7 +473,6 @@ struct pf_handle {
   float* dkf = nullptr;
   KTab kt{};
   KTabF ktf{};
-  double* dgam = nullptr;
 
   // faithful (FP64) banks
   Bank<double> fs{}, fgrad{}, fdfdc{}, fdum{}, fpsi{};
@@ -655,11 +662,11 @@ int step_faithful(pf_handle* h, bool diag, long rec) {
   tbeg(h, ST_NONLINEAR);
   const size_t shb = sizeof(double) * ((size_t)g.var * NB + NB);
   if (diag)
-    hipLaunchKernelGGL((k_nl_faithful<true>), dim3(grid_for(g.P, NB)), dim3(NB), shb, s, g, h->u, h->dgam,
-                       st, (double*)h->fdfdc.main, (double*)h->fpsi.main, h->fmob, h->part_nl);
+    hipLaunchKernelGGL((k_nl_faithful<true>), dim3(grid_for(g.P, NB)), dim3(NB), shb, s, g, h->u, st,
+                       (double*)h->fdfdc.main, (double*)h->fpsi.main, h->fmob, h->part_nl);
   else
-    hipLaunchKernelGGL((k_nl_faithful<false>), dim3(grid_for(g.P, NB)), dim3(NB), shb, s, g, h->u,
-                       h->dgam, st, (double*)h->fdfdc.main, (double*)h->fpsi.main, h->fmob, h->part_nl);
+    hipLaunchKernelGGL((k_nl_faithful<false>), dim3(grid_for(g.P, NB)), dim3(NB), shb, s, g, h->u, st,
+                       (double*)h->fdfdc.main, (double*)h->fpsi.main, h->fmob, h->part_nl);
   tend(h);
   // loops 1-4: F(phi_i), F(c); gradients; f_int
   tbeg(h, ST_X_R2C);
@@ -1093,11 +1100,6 @@ extern "C" int pf_create(const pf_params* p, int nx, int ny, int nz, int var, in
     h->ktf.az = df; h->ktf.azp = df + nz;
   }
   if (mode == PF_MODE_FP64_FAITHFUL) {
-    std::vector<double> gam((size_t)var * var);
-    pf_gamma(var, p->sigma_1, p->sigma_2, gam.data());
-    if ((rc = dalloc(h, (void**)&h->dgam, sizeof(double) * gam.size()))) return bail(rc);
-    if (hipMemcpy(h->dgam, gam.data(), sizeof(double) * gam.size(), hipMemcpyHostToDevice) != hipSuccess)
-      return bail(fail(PF_EHIP, "copy of gamma failed"));
     if ((rc = alloc_bank<double>(h, h->fs, g.F)) || (rc = alloc_bank<double>(h, h->fgrad, 3 * g.F)) ||
         (rc = alloc_bank<double>(h, h->fdfdc, 1)) || (rc = alloc_bank<double>(h, h->fdum, 1)) ||
         (rc = alloc_bank<double>(h, h->fpsi, var)) || (rc = dalloc(h, (void**)&h->fmob, sizeof(double) * g.P)))
@@ -1381,7 +1383,7 @@ extern "C" int pf_diagnose(pf_handle* h, pf_diag* out) {
   const unsigned gr_nl = grid_for(g.P, NB);
   if (h->mode == PF_MODE_FP64_FAITHFUL) {
     const size_t shb = sizeof(double) * ((size_t)g.var * NB + NB);
-    hipLaunchKernelGGL((k_nl_faithful<true>), dim3(gr_nl), dim3(NB), shb, s, g, h->u, h->dgam,
+    hipLaunchKernelGGL((k_nl_faithful<true>), dim3(gr_nl), dim3(NB), shb, s, g, h->u,
                        (const double*)h->fs.main, (double*)h->fdfdc.main, (double*)h->fpsi.main, h->fmob,
                        h->part_nl);
     h->n_part_nl = gr_nl;
