@@ -61,9 +61,11 @@ template <bool MAX> __device__ __forceinline__ double bext(double v, double* red
   return s;
 }
 
-// Nonlinear terms for one point, reduced O(var) form (gamma block structure of
-// src/var_diff.f90:206-218).  v: field 0 = c, fields 1..V = eta, at stride fs.
-// Per matrix field the derivative
+// Nonlinear terms for two neighbouring points (x = 2k, 2k+1: the two halves of
+// one complex LDS element per field), reduced O(var) form (gamma block
+// structure of src/var_diff.f90:206-218).  v: field 0 = c, fields 1..V = eta,
+// at a stride of fs complex elements; the two points' arithmetic is independent
+// and interleaved.  Per matrix field the derivative
 //   eta_i (a_i - 1) + 2 eta_i (s1g (sm - a_i) + s2g av) + cT4m eta_i a_i,  a_i = eta_i^2
 // is evaluated as eta_i (alpha a_i + beta) with the point constants
 //   alpha = 1 - 2 s1g + cT4m,  beta = 2 (s1g sm + s2g av) - 1,
@@ -72,52 +74,68 @@ template <bool MAX> __device__ __forceinline__ double bext(double v, double* red
 // derivative by a few ulp of beta (the derivative enters the update multiplied
 // by dt L).  Extremes are tracked in the working precision (exact: they are
 // values of the field); the two energy sums in FP64.
-template <typename R, bool DIAG, bool WRITE>
-__device__ __forceinline__ void nl_point(R* v, long fs, int V, const UPar& u, R& mob_out,
-                                         double& a_ch, double& a_eta, R& cmax, R& cmin,
-                                         R& pmin, R& p3) {
+template <typename R, bool DIAG>
+__device__ __forceinline__ void nl_pair(c2<R>* v, long fs, int V, const UPar& u, R* mo, double& a_ch,
+                                        double& a_eta, R& cmax, R& cmin, R& pmin, R& p3) {
   const R am = (R)u.am, ap = (R)u.ap, cm = (R)u.cm, cp = (R)u.cp, s1g = (R)u.s1g, s2g = (R)u.s2g;
   const int mat = V - 1;
-  const R con = v[0];
-  R sm = 0, num = 0, pair = 0, fe = 0;
+  const c2<R> c0 = v[0];
+  const R con[2] = {c0.x, c0.y};
+  R sm[2] = {0, 0}, num[2] = {0, 0}, pair[2] = {0, 0}, fe[2] = {0, 0};
 #pragma unroll 2
   for (int i = 1; i <= mat; ++i) {
-    const R fi = v[i * fs];
-    const R a = fi * fi;
-    if (DIAG) pair += a * sm;
-    sm += a;
-    num += a * a;
-    if (DIAG) fe += a * (R(0.25) * a - R(0.5));
-  }
-  const R fv = v[V * fs], av = fv * fv, av2 = av * av;
-  const R d4 = num + av2;
-  const R inv_d4 = R(1) / d4;
-  const R hm = num * inv_d4, hp = R(1) - hm;
-  const R dm = con - cm, dp = con - cp;
-  const R fm = am * (dm * dm), fpp = ap * (dp * dp);
-  if (WRITE) {
-    v[0] = R(2) * (hm * am * dm + hp * ap * dp);
-    const R cT4m = R(4) * (fm - fpp) * (av2 * inv_d4) * inv_d4;
-    const R alpha = (R(1) - R(2) * s1g) + cT4m;
-    const R beta = R(2) * (s1g * sm + s2g * av) - R(1);
-#pragma unroll 4
-    for (int i = 1; i <= mat; ++i) {
-      const R fi = v[i * fs];
-      v[i * fs] = fi * (fi * fi * alpha + beta);
+    const c2<R> f = v[i * fs];
+    const R fi[2] = {f.x, f.y};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const R a = fi[h] * fi[h];
+      if (DIAG) pair[h] += a * sm[h];
+      sm[h] += a;
+      num[h] += a * a;
+      if (DIAG) fe[h] += a * (R(0.25) * a - R(0.5));
     }
-    const R gam = R(1) + R(4) * hm * (fpp - fm) * inv_d4;
-    v[V * fs] = fv * (av * gam + (R(2) * s2g * sm - R(1)));
   }
-  const R ps = sm + av;
-  mob_out = R(2) * (R(1) - ps) * (R)u.Dgb;
-  if (DIAG) {
-    fe += av * (R(0.25) * av - R(0.5)) + s1g * pair + s2g * av * sm;
-    a_ch += (double)(hm * fm + hp * fpp);
-    a_eta += (double)fe;
-    if (fv > p3) p3 = fv;
-    if (con <= cmin) cmin = con;
-    if (con > cmax) cmax = con;
-    if (ps < pmin) pmin = ps;
+  const c2<R> fvc = v[V * fs];
+  const R fv[2] = {fvc.x, fvc.y};
+  R alpha[2], beta[2], gam[2], hm[2], hp[2], fm[2], fpp[2], av[2], dfdc[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    av[h] = fv[h] * fv[h];
+    const R av2 = av[h] * av[h];
+    const R d4 = num[h] + av2;
+    const R inv_d4 = R(1) / d4;
+    hm[h] = num[h] * inv_d4;
+    hp[h] = R(1) - hm[h];
+    const R dm = con[h] - cm, dp = con[h] - cp;
+    fm[h] = am * (dm * dm);
+    fpp[h] = ap * (dp * dp);
+    dfdc[h] = R(2) * (hm[h] * am * dm + hp[h] * ap * dp);
+    const R cT4m = R(4) * (fm[h] - fpp[h]) * (av2 * inv_d4) * inv_d4;
+    alpha[h] = (R(1) - R(2) * s1g) + cT4m;
+    beta[h] = R(2) * (s1g * sm[h] + s2g * av[h]) - R(1);
+    gam[h] = R(1) + R(4) * hm[h] * (fpp[h] - fm[h]) * inv_d4;
+  }
+  v[0] = mk<R>(dfdc[0], dfdc[1]);
+#pragma unroll 2
+  for (int i = 1; i <= mat; ++i) {
+    const c2<R> f = v[i * fs];
+    v[i * fs] = mk<R>(f.x * (f.x * f.x * alpha[0] + beta[0]), f.y * (f.y * f.y * alpha[1] + beta[1]));
+  }
+  v[V * fs] = mk<R>(fv[0] * (av[0] * gam[0] + (R(2) * s2g * sm[0] - R(1))),
+                    fv[1] * (av[1] * gam[1] + (R(2) * s2g * sm[1] - R(1))));
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const R ps = sm[h] + av[h];
+    mo[h] = R(2) * (R(1) - ps) * (R)u.Dgb;
+    if (DIAG) {
+      fe[h] += av[h] * (R(0.25) * av[h] - R(0.5)) + s1g * pair[h] + s2g * av[h] * sm[h];
+      a_ch += (double)(hm[h] * fm[h] + hp[h] * fpp[h]);
+      a_eta += (double)fe[h];
+      if (fv[h] > p3) p3 = fv[h];
+      if (con[h] <= cmin) cmin = con[h];
+      if (con[h] > cmax) cmax = con[h];
+      if (ps < pmin) pmin = ps;
+    }
   }
 }
 
@@ -240,21 +258,26 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       if (rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
     __syncthreads();
     // 4. pointwise: point x of line l is component x%2 of element x/2 of row
-    //    f*LINES + l
+    //    f*LINES + l; each thread takes the two points of one element (256
+    //    points of a line are 128 pairs, one pass of a 192-thread block instead
+    //    of two passes for the first wave: measured 0.645 -> 0.616 ms)
     {
-      R* rl = reinterpret_cast<R*>(sh);
-      constexpr long fs = 2L * LINES * MP;  // field stride in reals
-      for (int p = tid; p < LINES * NX; p += nt) {
-        const int l = p / NX, x = p - l * NX;
-        R* vv = rl + 2L * l * MP + x;
-        const long gp = (L0 + l) * NX + x;  // point index within a field
+      constexpr long fs = (long)LINES * MP;  // field stride in complex elements
+      for (int p = tid; p < LINES * M; p += nt) {
+        const int l = p / M, k = p - l * M;
+        c2<R>* vv = sh + (long)l * MP + k;
+        const long gp = (L0 + l) * NX + 2 * k;  // point index within a field
         if (XONLY) {
-          vv[0] = vv[0] * mobd[gp];
+          const c2<R> a = vv[0];
+          vv[0] = mk<R>(a.x * mobd[gp], a.y * mobd[gp + 1]);
         } else {
-          R mo;
-          if (diag) nl_point<R, true, true>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
-          else nl_point<R, false, true>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
-          if (mob) mobd[gp] = mo;
+          R mo[2];
+          if (diag) nl_pair<R, true>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+          else nl_pair<R, false>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+          if (mob) {
+            mobd[gp] = mo[0];
+            mobd[gp + 1] = mo[1];
+          }
         }
       }
     }
