@@ -61,11 +61,12 @@ template <bool MAX> __device__ __forceinline__ double bext(double v, double* red
   return s;
 }
 
-// Nonlinear terms for two neighbouring points (x = 2k, 2k+1: the two halves of
-// one complex LDS element per field), reduced O(var) form (gamma block
-// structure of src/var_diff.f90:206-218).  v: field 0 = c, fields 1..V = eta,
-// at a stride of fs complex elements; the two points' arithmetic is independent
-// and interleaved.  Per matrix field the derivative
+// Nonlinear terms for NPT = 1 point or NPT = 2 neighbouring points (x = 2k,
+// 2k+1: the two halves of one complex LDS element per field), reduced O(var)
+// form (gamma block structure of src/var_diff.f90:206-218).  v: field 0 = c,
+// fields 1..V = eta, at a stride of fs elements (reals for one point, complex
+// for two); the points' arithmetic is independent and interleaved.  Per matrix
+// field the derivative
 //   eta_i (a_i - 1) + 2 eta_i (s1g (sm - a_i) + s2g av) + cT4m eta_i a_i,  a_i = eta_i^2
 // is evaluated as eta_i (alpha a_i + beta) with the point constants
 //   alpha = 1 - 2 s1g + cT4m,  beta = 2 (s1g sm + s2g av) - 1,
@@ -74,32 +75,49 @@ template <bool MAX> __device__ __forceinline__ double bext(double v, double* red
 // derivative by a few ulp of beta (the derivative enters the update multiplied
 // by dt L).  Extremes are tracked in the working precision (exact: they are
 // values of the field); the two energy sums in FP64.
-template <typename R, bool DIAG>
-__device__ __forceinline__ void nl_pair(c2<R>* v, long fs, int V, const UPar& u, R* mo, double& a_ch,
-                                        double& a_eta, R& cmax, R& cmin, R& pmin, R& p3) {
+template <typename R, int NPT> struct Pts;
+template <typename R> struct Pts<R, 1> {
+  using T = R;
+  static __device__ __forceinline__ R get(const R& v, int) { return v; }
+  static __device__ __forceinline__ R make(const R* a) { return a[0]; }
+};
+template <typename R> struct Pts<R, 2> {
+  using T = c2<R>;
+  static __device__ __forceinline__ R get(const c2<R>& v, int h) { return h ? v.y : v.x; }
+  static __device__ __forceinline__ c2<R> make(const R* a) { return mk<R>(a[0], a[1]); }
+};
+
+template <typename R, int NPT, bool DIAG>
+__device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, int V, const UPar& u, R* mo,
+                                          double& a_ch, double& a_eta, R& cmax, R& cmin, R& pmin, R& p3) {
+  using P = Pts<R, NPT>;
   const R am = (R)u.am, ap = (R)u.ap, cm = (R)u.cm, cp = (R)u.cp, s1g = (R)u.s1g, s2g = (R)u.s2g;
   const int mat = V - 1;
-  const c2<R> c0 = v[0];
-  const R con[2] = {c0.x, c0.y};
-  R sm[2] = {0, 0}, num[2] = {0, 0}, pair[2] = {0, 0}, fe[2] = {0, 0};
+  R con[NPT], sm[NPT], num[NPT], pair[NPT], fe[NPT], fv[NPT];
+  {
+    const typename P::T c0 = v[0], f0 = v[V * fs];
+#pragma unroll
+    for (int h = 0; h < NPT; ++h) {
+      con[h] = P::get(c0, h);
+      fv[h] = P::get(f0, h);
+      sm[h] = num[h] = pair[h] = fe[h] = R(0);
+    }
+  }
 #pragma unroll 2
   for (int i = 1; i <= mat; ++i) {
-    const c2<R> f = v[i * fs];
-    const R fi[2] = {f.x, f.y};
+    const typename P::T f = v[i * fs];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const R a = fi[h] * fi[h];
+    for (int h = 0; h < NPT; ++h) {
+      const R fi = P::get(f, h), a = fi * fi;
       if (DIAG) pair[h] += a * sm[h];
       sm[h] += a;
       num[h] += a * a;
       if (DIAG) fe[h] += a * (R(0.25) * a - R(0.5));
     }
   }
-  const c2<R> fvc = v[V * fs];
-  const R fv[2] = {fvc.x, fvc.y};
-  R alpha[2], beta[2], gam[2], hm[2], hp[2], fm[2], fpp[2], av[2], dfdc[2];
+  R alpha[NPT], beta[NPT], gam[NPT], hm[NPT], hp[NPT], fm[NPT], fpp[NPT], av[NPT], out[NPT];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < NPT; ++h) {
     av[h] = fv[h] * fv[h];
     const R av2 = av[h] * av[h];
     const R d4 = num[h] + av2;
@@ -109,22 +127,29 @@ __device__ __forceinline__ void nl_pair(c2<R>* v, long fs, int V, const UPar& u,
     const R dm = con[h] - cm, dp = con[h] - cp;
     fm[h] = am * (dm * dm);
     fpp[h] = ap * (dp * dp);
-    dfdc[h] = R(2) * (hm[h] * am * dm + hp[h] * ap * dp);
+    out[h] = R(2) * (hm[h] * am * dm + hp[h] * ap * dp);
     const R cT4m = R(4) * (fm[h] - fpp[h]) * (av2 * inv_d4) * inv_d4;
     alpha[h] = (R(1) - R(2) * s1g) + cT4m;
     beta[h] = R(2) * (s1g * sm[h] + s2g * av[h]) - R(1);
     gam[h] = R(1) + R(4) * hm[h] * (fpp[h] - fm[h]) * inv_d4;
   }
-  v[0] = mk<R>(dfdc[0], dfdc[1]);
+  v[0] = P::make(out);
 #pragma unroll 2
   for (int i = 1; i <= mat; ++i) {
-    const c2<R> f = v[i * fs];
-    v[i * fs] = mk<R>(f.x * (f.x * f.x * alpha[0] + beta[0]), f.y * (f.y * f.y * alpha[1] + beta[1]));
-  }
-  v[V * fs] = mk<R>(fv[0] * (av[0] * gam[0] + (R(2) * s2g * sm[0] - R(1))),
-                    fv[1] * (av[1] * gam[1] + (R(2) * s2g * sm[1] - R(1))));
+    const typename P::T f = v[i * fs];
+    R o[NPT];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < NPT; ++h) {
+      const R fi = P::get(f, h);
+      o[h] = fi * (fi * fi * alpha[h] + beta[h]);
+    }
+    v[i * fs] = P::make(o);
+  }
+#pragma unroll
+  for (int h = 0; h < NPT; ++h) out[h] = fv[h] * (av[h] * gam[h] + (R(2) * s2g * sm[h] - R(1)));
+  v[V * fs] = P::make(out);
+#pragma unroll
+  for (int h = 0; h < NPT; ++h) {
     const R ps = sm[h] + av[h];
     mo[h] = R(2) * (R(1) - ps) * (R)u.Dgb;
     if (DIAG) {
@@ -258,26 +283,39 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       if (rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
     __syncthreads();
     // 4. pointwise: point x of line l is component x%2 of element x/2 of row
-    //    f*LINES + l; each thread takes the two points of one element (256
-    //    points of a line are 128 pairs, one pass of a 192-thread block instead
-    //    of two passes for the first wave: measured 0.645 -> 0.616 ms)
-    {
+    //    f*LINES + l.  When the group has more points than the block has threads
+    //    (256 points, 192 threads at 256^3 x 17), each thread takes the two
+    //    points of one element: one pass instead of two for the first wave
+    //    (measured 0.645 -> 0.616 ms); otherwise one point per thread (pairs
+    //    measured 4% slower at 128^3, where the points fit one pass anyway).
+    if (!XONLY && LINES * NX > nt) {
       constexpr long fs = (long)LINES * MP;  // field stride in complex elements
       for (int p = tid; p < LINES * M; p += nt) {
         const int l = p / M, k = p - l * M;
         c2<R>* vv = sh + (long)l * MP + k;
         const long gp = (L0 + l) * NX + 2 * k;  // point index within a field
+        R mo[2];
+        if (diag) nl_points<R, 2, true>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+        else nl_points<R, 2, false>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+        if (mob) {
+          mobd[gp] = mo[0];
+          mobd[gp + 1] = mo[1];
+        }
+      }
+    } else {
+      R* rl = reinterpret_cast<R*>(sh);
+      constexpr long fs = 2L * LINES * MP;  // field stride in reals
+      for (int p = tid; p < LINES * NX; p += nt) {
+        const int l = p / NX, x = p - l * NX;
+        R* vv = rl + 2L * l * MP + x;
+        const long gp = (L0 + l) * NX + x;
         if (XONLY) {
-          const c2<R> a = vv[0];
-          vv[0] = mk<R>(a.x * mobd[gp], a.y * mobd[gp + 1]);
+          vv[0] = vv[0] * mobd[gp];
         } else {
-          R mo[2];
-          if (diag) nl_pair<R, true>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
-          else nl_pair<R, false>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
-          if (mob) {
-            mobd[gp] = mo[0];
-            mobd[gp + 1] = mo[1];
-          }
+          R mo[1];
+          if (diag) nl_points<R, 1, true>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+          else nl_points<R, 1, false>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+          if (mob) mobd[gp] = mo[0];
         }
       }
     }
