@@ -417,12 +417,15 @@ __device__ __forceinline__ void neum(double& s, double& c, double x) {
 }
 // out: the row, or (ctr != null) the base of the rows: row *ctr is written and
 // *ctr advanced by ctr_step, so that a captured step writes each replay's row.
-__global__ void __launch_bounds__(NB) k_diag_final(DiagSrc d, double* __restrict__ out, int* __restrict__ ctr,
+// One block of kDiagThreads: the partials of a 256^3 step (8192 x 6 from the
+// x line, 32768 from the z line) are read by 1024 threads instead of 256.
+constexpr int kDiagThreads = 1024;
+__global__ void __launch_bounds__(kDiagThreads) k_diag_final(DiagSrc d, double* __restrict__ out, int* __restrict__ ctr,
                                                    int ctr_step) {
-  __shared__ double red[NB];
+  __shared__ double red[kDiagThreads];
   double sch = 0, cch = 0, set = 0, cet = 0, sfi = 0, cfi = 0;
   double cmax = 0.0, cmin = 1.0, pmin = 1.0, p3 = 0.0;
-  for (long b = threadIdx.x; b < d.n_nl; b += NB) {
+  for (long b = threadIdx.x; b < d.n_nl; b += kDiagThreads) {
     const double* o = d.nl + b * NPART_NL;
     neum(sch, cch, o[0]);
     neum(set, cet, o[1]);
@@ -431,12 +434,12 @@ __global__ void __launch_bounds__(NB) k_diag_final(DiagSrc d, double* __restrict
     if (o[4] < pmin) pmin = o[4];
     if (o[5] > p3) p3 = o[5];
   }
-  for (long b = threadIdx.x; b < d.n_fi; b += NB) neum(sfi, cfi, d.fi[b]);
-  const double fch = block_sum<NB>(sch + cch, red);
-  const double feta = block_sum<NB>(set + cet, red);
-  const double fint = block_sum<NB>(sfi + cfi, red);
-  const double e0 = block_ext<NB, true>(cmax, red), e1 = block_ext<NB, false>(cmin, red);
-  const double e2 = block_ext<NB, false>(pmin, red), e3 = block_ext<NB, true>(p3, red);
+  for (long b = threadIdx.x; b < d.n_fi; b += kDiagThreads) neum(sfi, cfi, d.fi[b]);
+  const double fch = block_sum<kDiagThreads>(sch + cch, red);
+  const double feta = block_sum<kDiagThreads>(set + cet, red);
+  const double fint = block_sum<kDiagThreads>(sfi + cfi, red);
+  const double e0 = block_ext<kDiagThreads, true>(cmax, red), e1 = block_ext<kDiagThreads, false>(cmin, red);
+  const double e2 = block_ext<kDiagThreads, false>(pmin, red), e3 = block_ext<kDiagThreads, true>(p3, red);
   if (threadIdx.x == 0) {
     if (ctr) {
       const int row = *ctr;
@@ -600,9 +603,9 @@ int diag_final(pf_handle* h, long idx, bool with_nl, bool with_fi, const double*
   d.cmean_f = cmean_f;
   d.cmean_is_float = cmean_f != nullptr;
   if (h->capturing)
-    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(NB), 0, h->s, d, h->ddiag, h->dctr, h->cap_step);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kDiagThreads), 0, h->s, d, h->ddiag, h->dctr, h->cap_step);
   else
-    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(NB), 0, h->s, d, h->ddiag + 9 * idx, (int*)nullptr, 0);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kDiagThreads), 0, h->s, d, h->ddiag + 9 * idx, (int*)nullptr, 0);
   return 0;
 }
 
