@@ -174,7 +174,7 @@ int x_r2c(int nx, c2<R>* data, c2<R>* nyq, long nlines, R scale, hipStream_t s) 
   switch (nx) {
 #define PF_CASE(N)                                                                      \
   case N: {                                                                             \
-    constexpr int LPB = lines_per_block(N / 2);                                         \
+    constexpr int LPB = x_lines<R>(N / 2);                                              \
     hipLaunchKernelGGL((k_x_r2c<R, N>), dim3(ceil_div(nlines, LPB)),                    \
                        dim3(LPB * plan_T(N / 2)), 0, s, data, nyq, nlines, tw, scale); \
     break;                                                                              \
@@ -193,7 +193,7 @@ int x_c2r(int nx, c2<R>* data, const c2<R>* nyq, long nlines, hipStream_t s) {
   switch (nx) {
 #define PF_CASE(N)                                                                   \
   case N: {                                                                          \
-    constexpr int LPB = lines_per_block(N / 2);                                      \
+    constexpr int LPB = x_lines<R>(N / 2);                                           \
     hipLaunchKernelGGL((k_x_c2r<R, N>), dim3(ceil_div(nlines, LPB)),                 \
                        dim3(LPB * plan_T(N / 2)), 0, s, data, nyq, nlines, tw); \
     break;                                                                           \
@@ -213,7 +213,7 @@ int x_c2r_sumsq(int nx, const c2<R>* data, const c2<R>* nyq, long nlines, long l
   switch (nx) {
 #define PF_CASE(N)                                                                                   \
   case N: {                                                                                          \
-    constexpr int LPB = lines_per_block(N / 2);                                                      \
+    constexpr int LPB = x_lines<R>(N / 2);                                                           \
     const long grid = std::max<long>(1, std::min<long>((nlines + LPB - 1) / LPB, max_blocks));       \
     hipLaunchKernelGGL((k_x_c2r_sumsq<R, N>), dim3((unsigned)grid), dim3(LPB * plan_T(N / 2)), 0, s, \
                        data, nyq, nlines, lines_per_field, w0, w1, tw, part);                        \
