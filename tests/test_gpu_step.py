@@ -57,6 +57,16 @@ def test_fp32_mixed_vs_oracle(pf, oracle, dgb):
     _compare(pf, oracle, (32, 32, 32), 4, 8, 20, "fp32_mixed", dict(VOR, D_gb=dgb), 1e-4, 1e-4)
 
 
+# One or two order parameters: the h sum runs over no field (var = 1) or one
+# (var = 2), the gamma matrix has no sigma_1 pair, and the x-line blocks hold 2
+# or 3 fields.
+@pytest.mark.parametrize("var", [1, 2])
+@pytest.mark.parametrize("mode,tol_s,tol_e", [("fp64_faithful", 1e-11, 1e-10), ("fp64_reduced", 1e-11, 1e-10),
+                                              ("fp32_mixed", 1e-4, 1e-4)])
+def test_few_order_parameters_vs_oracle(pf, oracle, var, mode, tol_s, tol_e):
+    _compare(pf, oracle, (16, 24, 16), var, 4, 10, mode, dict(VOR, D_gb=0.3), tol_s, tol_e)
+
+
 @pytest.mark.parametrize("mode", ["fp64_faithful", "fp64_reduced", "fp32_mixed"])
 def test_diagnose_matches_next_step_record(pf, mode):
     """pf_diagnose reports the start-of-step diagnostics of the current state, i.e.
