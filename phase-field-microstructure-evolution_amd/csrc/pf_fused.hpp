@@ -429,7 +429,7 @@ __device__ __forceinline__ double recip_fast(double x) {
 }
 
 // Minimum waves per SIMD asked of the register allocator: FP32 fits 3 waves
-// (158 / 168 VGPRs with and without the Parseval sums, no spills), FP64 2.
+// (158 VGPRs for the plain update, 168 with the Parseval sums, no spills), FP64 2.
 template <typename R, bool PARSEVAL> constexpr int kZlineWaves = sizeof(R) == 4 ? 3 : 2;
 
 template <typename R, int NZ, int ROLE, bool DC64, bool PARSEVAL>
