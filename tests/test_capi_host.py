@@ -67,3 +67,15 @@ def test_errors_are_reported_not_crashing(pf):
     assert e.value.code == pf.PF_EINVAL
     with pytest.raises(pf.PfError):
         pf.k_space(1, 8, 8)
+
+
+def test_missing_library_fails_loudly(pf, monkeypatch):
+    """Without libpf.so there is no CPU path: every entry point raises."""
+    monkeypatch.setattr(pf, "_lib", None)
+    monkeypatch.setattr(pf, "LIB_PATH", os.path.join(ROOT, "no_such_dir", "libpf.so"))
+    with pytest.raises(ImportError):
+        pf.lib()
+    with pytest.raises(ImportError):
+        pf.Simulation(pf.make_params(), 16, 16, 16, 3, "fp32_mixed")
+    with pytest.raises(ImportError):
+        pf.k_space(8, 8, 8)
