@@ -91,13 +91,19 @@ def roofline_of(stages, steps):
     return name, avg_ms, achieved, st["bytes"]
 
 
-# Kernel behind each timed stage of the FP32 step at 256^3 (names as rocprofv3 prints them).
-STAGE_KERNELS = {
-    "nonlinear": ["pf::k_xline<float, 256, 1, false>"],
-    "y_fwd": ["pf::k_ypass<float, 256, false>"],
-    "kspace_update": ["pf::k_zline<float, 256, 0, true, false>", "pf::k_zline<float, 256, 0, true, true>"],
-    "y_inv": ["pf::k_ypass<float, 256, true>"],
-}
+def stage_kernels(n):
+    """Kernel behind each timed stage of the FP32 step on an n^3 grid (names as
+    rocprofv3 prints them; the x-line's lines-per-block argument is shown only
+    at 256^3, where it is 1)."""
+    return {
+        "nonlinear": [f"pf::k_xline<float, {n}, 1, false>" if n == 256 else f"pf::k_xline<float, {n}, ...>"],
+        "y_fwd": [f"pf::k_ypass<float, {n}, false>"],
+        "kspace_update": [f"pf::k_zline<float, {n}, 0, true, false>", f"pf::k_zline<float, {n}, 0, true, true>"],
+        "y_inv": [f"pf::k_ypass<float, {n}, true>"],
+    }
+
+
+STAGE_KERNELS = stage_kernels(256)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fused_kernels_256x17_fp32.md")
 
 
@@ -351,7 +357,7 @@ def main():
         "field_updates_per_s": gpups * F,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name, n, var),
-                     "stage": name, "kernel": " / ".join(STAGE_KERNELS.get(name, [name])),
+                     "stage": name, "kernel": " / ".join(stage_kernels(n).get(name, [name])),
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": kbytes,
                      "stream_reference": {"achieved": stream, "frac": achieved / stream if stream else None,
                                           "kernel": "torch in-place multiply over the same bytes "
