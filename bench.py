@@ -122,20 +122,61 @@ def measured_traffic(stage, n, var):
     return sum(vals) / len(vals)
 
 
-def cpu_baseline(n, var, phi, con, nsteps):
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads() -> int:
+    """The host threads the CPU leg uses: OMP_NUM_THREADS when the environment
+    sets it (the GPU box does: its CPU share per GPU), else every core in this
+    process's affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(pf, n, var, phi, con, nsteps):
+    """The oracle (the reference step restated in C, FP64, the reference's
+    operation sequence) on the host cores: one untimed warm-up step on a 64^3
+    grid, then nsteps timed steps of the benchmark workload with cpu_threads()
+    threads, and one step of a 96^3 grid of the same workload on one thread."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
-    oracle.set_threads(threads)
+    threads = cpu_threads()
     p = oracle.make_params(t_step=0.005, grad_coeff_phi=1.0, grad_coeff_c=1.0, D_gb=0.0)
+    wphi, wcon = pf.ic_voronoi(64, 64, 64, var, 8, -1, 0.4, 0.02)
+    oracle.set_threads(threads)
+    oracle.run(p, wphi, wcon, 1)
     ophi, ocon = phi.copy(), con.copy()
     t0 = time.perf_counter()
     oracle.run(p, ophi, ocon, nsteps)
     dt = time.perf_counter() - t0
+    del ophi, ocon
+    n1 = 96
+    sphi, scon = pf.ic_voronoi(n1, n1, n1, var, 8, -1, 0.4, 0.02)
+    oracle.set_threads(1)
+    t1 = time.perf_counter()
+    oracle.run(p, sphi, scon, 1)
+    dt1 = time.perf_counter() - t1
+    oracle.set_threads(threads)
     return dict(value=n ** 3 * nsteps / dt, unit="grid-point-updates/s", cores=threads,
                 kind="port",
                 sample=f"{nsteps} FP64 faithful steps of the same {n}^3 x {var + 1} workload "
-                       f"({dt:.1f} s), oracle/pf_oracle.c with {threads} OpenMP threads")
+                       f"({dt:.1f} s) after a 64^3 warm-up step, oracle/pf_oracle.c with {threads} "
+                       f"OpenMP threads",
+                host={"cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                      "affinity_cores": len(os.sched_getaffinity(0)),
+                      "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")},
+                one_thread={"value": n1 ** 3 / dt1, "unit": "grid-point-updates/s",
+                            "sample": f"1 FP64 faithful step of a {n1}^3 x {var + 1} Voronoi grid "
+                                      f"({dt1:.1f} s) on 1 thread"})
 
 
 def accuracy(pf, n, var, phi, con, args):
@@ -328,8 +369,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: {args.cpu_steps} oracle steps at {n}^3 x {var + 1}")
-        cpu = cpu_baseline(n, var, phi, con, args.cpu_steps)
-        log(f"cpu baseline: {cpu['value']:.3e} GPUPS")
+        cpu = cpu_baseline(pf, n, var, phi, con, args.cpu_steps)
+        log(f"cpu baseline: {cpu['value']:.3e} GPUPS on {cpu['cores']} threads "
+            f"({cpu['host']['cpu_model']}), 1 thread {cpu['one_thread']['value']:.3e}")
 
     # whole-step HBM model (reduced, constant mobility): F(4 B_r + 13 B_c) per step
     F = var + 1

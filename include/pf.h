@@ -72,6 +72,13 @@ int pf_destroy(pf_handle* h);
 /* Upload / download the state (src/var_diff.f90:154,143 layouts). */
 int pf_put_state(pf_handle* h, const double* phi, const double* con);
 int pf_get_state(pf_handle* h, double* phi, double* con);
+/* The Voronoi polycrystal of pf_ic_voronoi (same draws, same owners), built on
+ * the device straight into the handle: equivalent to pf_ic_voronoi followed by
+ * pf_put_state, without the host phi array (no counterpart in the reference,
+ * whose polycrystal start reads a grain-map file, src/var_diff.f90:232-252;
+ * this is the synthetic start of the benchmark workloads at sizes where the
+ * host array would be tens of GB).  nseeds in [1, 65536]. */
+int pf_put_voronoi(pf_handle* h, int nseeds, int iseed, double c0, double c_amp);
 /* Advance nsteps steps.  out (nullable) receives nsteps records; a record is
  * filled when its step number is a multiple of diag_every (diag_every <= 0: none).
  * Returns PF_ENONFINITE if a diagnosed energy or mean is NaN/Inf. */
@@ -107,7 +114,9 @@ void* pf_stream(pf_handle* h);
  * `layout`, multiplied by `scale` (1/(nx*ny*nz) reproduces f_trans).
  * pf_fft_c2r: inverse C2R, unscaled (inv_trans); imaginary parts of the kx=0
  * and kx=nx/2 entries are ignored.  prec = PF_PREC_FP64 / PF_PREC_FP32.
- * stream: hipStream_t or NULL. */
+ * stream: hipStream_t or NULL.  Both share one scratch buffer per device: calls
+ * from several threads or streams take turns on it in stream order (each call's
+ * work waits for the previous user's), so they are safe but do not overlap. */
 int pf_fft_r2c(int nx, int ny, int nz, int batch, int prec, const void* d_in, void* d_out,
                int layout, double scale, void* stream);
 int pf_fft_c2r(int nx, int ny, int nz, int batch, int prec, const void* d_in, void* d_out,

@@ -542,10 +542,12 @@ int ora_run(const ora_params* p, int nx, int ny, int nz, int var,
     double f_int = 0.0;
     /* loops 1-2: gradient energy of each order parameter */
     for (int iv = 0; iv < var; ++iv) {
+#pragma omp parallel for schedule(static)
       for (size_t t = 0; t < P; ++t) dummy[t] = phi[t * var + iv];
       ora_fft_r2c(nx, ny, nz, dummy, (double*)spec, 1);
       double part = 0.0;
       for (int d = 0; d < 3; ++d) {
+#pragma omp parallel for schedule(static)
         for (size_t t = 0; t < H; ++t) {
           double a = kap[3 * t + d];
           sgrad[t].r = -(spec[t].i * a);
@@ -555,8 +557,14 @@ int ora_run(const ora_params* p, int nx, int ny, int nz, int var,
         /* grad_x, grad_y, grad_z are kept per point in the reference and
          * summed as ((gx^2 + gy^2) + gz^2); accumulate the same way. */
         if (d == 0) memcpy(dummy, gradr, sizeof(double) * P);
-        else if (d == 1) for (size_t t = 0; t < P; ++t) dummy[t] = dummy[t] * dummy[t] + gradr[t] * gradr[t];
-        else for (size_t t = 0; t < P; ++t) dummy[t] = dummy[t] + gradr[t] * gradr[t];
+        else if (d == 1) {
+#pragma omp parallel for schedule(static)
+          for (size_t t = 0; t < P; ++t) dummy[t] = dummy[t] * dummy[t] + gradr[t] * gradr[t];
+        }
+        else {
+#pragma omp parallel for schedule(static)
+          for (size_t t = 0; t < P; ++t) dummy[t] = dummy[t] + gradr[t] * gradr[t];
+        }
       }
       (void)part;
 #pragma omp parallel for schedule(static)
@@ -571,6 +579,7 @@ int ora_run(const ora_params* p, int nx, int ny, int nz, int var,
     /* loops 3-4: gradient energy of c */
     ora_fft_r2c(nx, ny, nz, con, (double*)dft_con, 1);
     for (int d = 0; d < 3; ++d) {
+#pragma omp parallel for schedule(static)
       for (size_t t = 0; t < H; ++t) {
         double a = kap[3 * t + d];
         sgrad[t].r = -(dft_con[t].i * a);
@@ -578,8 +587,14 @@ int ora_run(const ora_params* p, int nx, int ny, int nz, int var,
       }
       ora_fft_c2r(nx, ny, nz, (double*)sgrad, gradr);
       if (d == 0) memcpy(dummy, gradr, sizeof(double) * P);
-      else if (d == 1) for (size_t t = 0; t < P; ++t) dummy[t] = dummy[t] * dummy[t] + gradr[t] * gradr[t];
-      else for (size_t t = 0; t < P; ++t) dummy[t] = dummy[t] + gradr[t] * gradr[t];
+      else if (d == 1) {
+#pragma omp parallel for schedule(static)
+        for (size_t t = 0; t < P; ++t) dummy[t] = dummy[t] * dummy[t] + gradr[t] * gradr[t];
+      }
+      else {
+#pragma omp parallel for schedule(static)
+        for (size_t t = 0; t < P; ++t) dummy[t] = dummy[t] + gradr[t] * gradr[t];
+      }
     }
 #pragma omp parallel for schedule(static)
     for (int z = 0; z < nz; ++z) {
@@ -624,6 +639,7 @@ int ora_run(const ora_params* p, int nx, int ny, int nz, int var,
     }
     ora_fft_c2r(nx, ny, nz, (double*)dft_dummy, dummy);
     /* loop 12 */
+#pragma omp parallel for schedule(static)
     for (size_t t = 0; t < P; ++t) dummy[t] = dummy[t] * (mob_c[t] - D);
     ora_fft_r2c(nx, ny, nz, dummy, (double*)dft_dummy, 1);
     /* loop 13: CH update */
@@ -642,8 +658,10 @@ int ora_run(const ora_params* p, int nx, int ny, int nz, int var,
     ora_fft_c2r(nx, ny, nz, (double*)dft_con, con);
     /* loop 14: AC update per order parameter (uses phi from before this loop) */
     for (int iv = 0; iv < var; ++iv) {
+#pragma omp parallel for schedule(static)
       for (size_t t = 0; t < P; ++t) dummy[t] = phi[t * var + iv];
       ora_fft_r2c(nx, ny, nz, dummy, (double*)spec, 1);
+#pragma omp parallel for schedule(static)
       for (size_t t = 0; t < P; ++t) dummy[t] = df_dphi[t * var + iv];
       ora_fft_r2c(nx, ny, nz, dummy, (double*)spec2, 1);
       const double ldt = p->mob_phi * p->t_step;
@@ -656,6 +674,7 @@ int ora_run(const ora_params* p, int nx, int ny, int nz, int var,
         spec[t].i = (spec[t].i - term11.i) / term22;
       }
       ora_fft_c2r(nx, ny, nz, (double*)spec, dummy);
+#pragma omp parallel for schedule(static)
       for (size_t t = 0; t < P; ++t) phi[t * var + iv] = dummy[t];
     }
     ora_extrema((int)P, var, phi, con, ext);

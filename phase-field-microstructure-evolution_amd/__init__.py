@@ -67,7 +67,7 @@ EXPORTED = (
     "pf_set_timing", "pf_stage_times", "pf_stage_name", "pf_stage_bytes", "pf_stream",
     "pf_fft_r2c", "pf_fft_c2r", "pf_f_trans", "pf_inv_trans", "pf_ran2_init", "pf_ran2",
     "pf_k_space", "pf_gamma", "pf_ic_random", "pf_ic_voronoi", "pf_last_error", "pf_version",
-    "pf_ensemble_step", "pf_analyze_state", "pf_set_graphs",
+    "pf_ensemble_step", "pf_analyze_state", "pf_set_graphs", "pf_put_voronoi",
 )
 
 
@@ -98,6 +98,7 @@ def lib():
     L.pf_destroy.argtypes = [vp]
     L.pf_put_state.argtypes = [vp, _dp, _dp]
     L.pf_get_state.argtypes = [vp, _dp, _dp]
+    L.pf_put_voronoi.argtypes = [vp, ip, ip, dp, dp]
     L.pf_step.argtypes = [vp, ip, ip, vp]
     L.pf_diagnose.argtypes = [vp, C.POINTER(Diag)]
     L.pf_set_timing.argtypes = [vp, ip]
@@ -184,6 +185,10 @@ class Simulation:
         assert con.shape == (self.nz, self.ny, self.nx), con.shape
         _check(lib().pf_put_state(self._h, np.ascontiguousarray(phi, dtype=np.float64),
                                   np.ascontiguousarray(con, dtype=np.float64)))
+
+    def put_voronoi(self, nseeds: int, iseed: int = -1, c0: float = 0.4, c_amp: float = 0.02) -> None:
+        """The state ic_voronoi(...) would give, built on the device (pf_put_voronoi)."""
+        _check(lib().pf_put_voronoi(self._h, nseeds, iseed, c0, c_amp))
 
     def get_state(self):
         phi = np.empty((self.nz, self.ny, self.nx, self.var))

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <string>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -81,8 +83,9 @@ struct EventPair {
 };
 }  // namespace
 
-int to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
-  std::lock_guard<std::mutex> lk(g_bounce_mu);
+namespace {
+// Under g_bounce_mu.
+int to_host_locked(void* dst, const void* src, size_t bytes, hipStream_t s) {
   EventPair ev;
   if (bytes < kBounceMin || !ev.ok || !bounce_ready()) {
     PF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
@@ -107,8 +110,7 @@ int to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
   return 0;
 }
 
-int to_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
-  std::lock_guard<std::mutex> lk(g_bounce_mu);
+int to_device_locked(void* dst, const void* src, size_t bytes, hipStream_t s) {
   EventPair ev;
   if (bytes < kBounceMin || !ev.ok || !bounce_ready()) {
     PF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
@@ -128,36 +130,99 @@ int to_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
   PF_HIP(hipStreamSynchronize(s));
   return 0;
 }
+}  // namespace
+
+// The bounce buffers are released with the lock: on an error part-way through,
+// the stream is drained first so that no copy into or out of them is still in
+// flight when the next transfer takes them.
+int to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_bounce_mu);
+  const int rc = to_host_locked(dst, src, bytes, s);
+  if (rc) (void)hipStreamSynchronize(s);
+  return rc;
+}
+
+int to_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_bounce_mu);
+  const int rc = to_device_locked(dst, src, bytes, s);
+  if (rc) (void)hipStreamSynchronize(s);
+  return rc;
+}
 
 // ---------------------------------------------------------------------------
 // Twiddle tables.
 // ---------------------------------------------------------------------------
 
+// Every length's table W_n^t = exp(-2 pi i t/n), t < n, back to back in the
+// order of PF_GRID_SIZES, one array per precision in static device memory.
+// Each device's copy is filled once (a host-to-device copy, no allocation), on
+// the first use of any table on that device.
 namespace {
+#define PF_SUM(N) +N
+constexpr int kTwTotal = 0 PF_GRID_SIZES(PF_SUM);
+#undef PF_SUM
+__device__ __align__(256) double2 g_tw64[kTwTotal];
+__device__ __align__(256) float2 g_tw32[kTwTotal];
+
+int tw_offset(int n) {
+  int off = 0;
+#define PF_OFF(N) \
+  if (n == N) return off; \
+  off += N;
+  PF_GRID_SIZES(PF_OFF)
+#undef PF_OFF
+  return -1;
+}
+
 std::mutex g_tw_mu;
-std::map<std::pair<int, int>, void*> g_tw[2];  // [precision][(device, n)]
+std::map<int, std::pair<double2*, float2*>> g_tw;  // device -> its copies of g_tw64, g_tw32
+
+// Under g_tw_mu: the device's table addresses, filling them on first use.
+bool tw_ready(int dev, std::pair<double2*, float2*>* out) {
+  auto it = g_tw.find(dev);
+  if (it != g_tw.end()) { *out = it->second; return true; }
+  void *p64 = nullptr, *p32 = nullptr;
+  if (hipGetSymbolAddress(&p64, HIP_SYMBOL(g_tw64)) != hipSuccess ||
+      hipGetSymbolAddress(&p32, HIP_SYMBOL(g_tw32)) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  std::vector<double2> h64(kTwTotal);
+  std::vector<float2> h32(kTwTotal);
+  int off = 0;
+  auto fill = [&](int n) {
+    std::vector<double> re(n), im(n);
+    host_twiddles(n, re.data(), im.data());
+    for (int t = 0; t < n; ++t) {
+      h64[off + t].x = re[t]; h64[off + t].y = im[t];
+      h32[off + t].x = (float)re[t]; h32[off + t].y = (float)im[t];
+    }
+    off += n;
+  };
+#define PF_FILL(N) fill(N);
+  PF_GRID_SIZES(PF_FILL)
+#undef PF_FILL
+  if (hipMemcpy(p64, h64.data(), sizeof(double2) * kTwTotal, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p32, h32.data(), sizeof(float2) * kTwTotal, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  *out = g_tw[dev] = std::make_pair((double2*)p64, (float2*)p32);
+  return true;
+}
 }  // namespace
 
 template <typename R> const c2<R>* twiddle_table(int n) {
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  const int pi = sizeof(R) == 8 ? 1 : 0;
-  std::lock_guard<std::mutex> lk(g_tw_mu);
-  auto key = std::make_pair(dev, n);
-  auto it = g_tw[pi].find(key);
-  if (it != g_tw[pi].end()) return static_cast<const c2<R>*>(it->second);
-  std::vector<double> re(n), im(n);
-  host_twiddles(n, re.data(), im.data());
-  std::vector<c2<R>> h(n);
-  for (int t = 0; t < n; ++t) { h[t].x = (R)re[t]; h[t].y = (R)im[t]; }
-  void* d = nullptr;
-  if (hipMalloc(&d, sizeof(c2<R>) * n) != hipSuccess) return nullptr;
-  if (hipMemcpy(d, h.data(), sizeof(c2<R>) * n, hipMemcpyHostToDevice) != hipSuccess) {
-    (void)hipFree(d);
-    return nullptr;
+  const int off = tw_offset(n);
+  if (off < 0 || hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::pair<double2*, float2*> t;
+  {
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    if (!tw_ready(dev, &t)) return nullptr;
   }
-  g_tw[pi][key] = d;
-  return static_cast<const c2<R>*>(d);
+  if constexpr (sizeof(R) == 8) return t.first + off;
+  else return t.second + off;
 }
 template const c2<float>* twiddle_table<float>(int);
 template const c2<double>* twiddle_table<double>(int);
@@ -170,7 +235,7 @@ static inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) 
 template <typename R>
 int x_r2c(int nx, c2<R>* data, c2<R>* nyq, long nlines, R scale, hipStream_t s) {
   const c2<R>* tw = twiddle_table<R>(nx);
-  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");
   switch (nx) {
 #define PF_CASE(N)                                                                      \
   case N: {                                                                             \
@@ -189,7 +254,7 @@ int x_r2c(int nx, c2<R>* data, c2<R>* nyq, long nlines, R scale, hipStream_t s) 
 template <typename R>
 int x_c2r(int nx, c2<R>* data, const c2<R>* nyq, long nlines, hipStream_t s) {
   const c2<R>* tw = twiddle_table<R>(nx);
-  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");
   switch (nx) {
 #define PF_CASE(N)                                                                   \
   case N: {                                                                          \
@@ -209,7 +274,7 @@ template <typename R>
 int x_c2r_sumsq(int nx, const c2<R>* data, const c2<R>* nyq, long nlines, long lines_per_field, double w0,
                 double w1, double* part, long max_blocks, long* nblocks, hipStream_t s) {
   const c2<R>* tw = twiddle_table<R>(nx);
-  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");
   switch (nx) {
 #define PF_CASE(N)                                                                                   \
   case N: {                                                                                          \
@@ -230,7 +295,7 @@ int x_c2r_sumsq(int nx, const c2<R>* data, const c2<R>* nyq, long nlines, long l
 template <typename R>
 int col_pass(int n, bool inv, c2<R>* data, long W, long npanels, hipStream_t s) {
   const c2<R>* tw = twiddle_table<R>(n);
-  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");
   switch (n) {
 #define PF_CASE(N)                                                                          \
   case N: {                                                                                 \
@@ -253,7 +318,7 @@ int col_pass(int n, bool inv, c2<R>* data, long W, long npanels, hipStream_t s) 
 template <typename R>
 int row_pass(int n, bool inv, c2<R>* data, long nlines, hipStream_t s) {
   const c2<R>* tw = twiddle_table<R>(n);
-  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");
   switch (n) {
 #define PF_CASE(N)                                                                      \
   case N: {                                                                             \
@@ -382,34 +447,85 @@ int check_dims(int nx, int ny, int nz, int batch) {
   return 0;
 }
 
+// Scratch of pf_fft_r2c / pf_fft_c2r: one buffer per device, shared by every
+// caller and stream.  A call holds the device's lock from taking the buffer
+// until its work is enqueued; that work first waits (on the caller's stream)
+// for the event the previous user recorded after its last use, and records the
+// event again afterwards, so uses on different streams or threads run one after
+// another on the GPU.  A buffer that must grow is freed only after that event
+// has completed.
 struct Workspace {
+  std::mutex mu;
   void* p = nullptr;
   size_t bytes = 0;
+  hipEvent_t done = nullptr;
 };
 std::mutex g_ws_mu;
-std::map<int, Workspace> g_ws;
+std::map<int, std::unique_ptr<Workspace>> g_ws;
 
-void* workspace(size_t bytes) {
+Workspace* device_workspace() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  Workspace& w = g_ws[dev];
-  if (w.bytes < bytes) {
-    if (w.p) (void)hipFree(w.p);
-    w.p = nullptr;
-    w.bytes = 0;
-    if (hipMalloc(&w.p, bytes) != hipSuccess) return nullptr;
-    w.bytes = bytes;
-  }
-  return w.p;
+  auto& w = g_ws[dev];
+  if (!w) w.reset(new Workspace);
+  return w.get();
 }
 
+// Holds the device workspace (locked) for one call on stream s.
+struct WorkspaceLease {
+  Workspace* w = nullptr;
+  std::unique_lock<std::mutex> lk;
+  hipStream_t s;
+  void* p = nullptr;
+  std::string err;
+  WorkspaceLease(size_t bytes, hipStream_t s_) : s(s_) {
+    w = device_workspace();
+    if (!w) { err = "no current device"; return; }
+    lk = std::unique_lock<std::mutex>(w->mu);
+    if (!w->done && hipEventCreateWithFlags(&w->done, hipEventDisableTiming) != hipSuccess) {
+      w->done = nullptr;
+      err = "event creation failed";
+      return;
+    }
+    if (w->bytes < bytes) {
+      if (w->p) {
+        if (hipEventSynchronize(w->done) != hipSuccess) { err = "waiting for the previous use failed"; return; }
+        (void)hipFree(w->p);
+      }
+      w->p = nullptr;
+      w->bytes = 0;
+      if (hipMalloc(&w->p, bytes) != hipSuccess) {
+        w->p = nullptr;
+        (void)hipGetLastError();
+        err = "workspace allocation of " + std::to_string(bytes) + " bytes failed";
+        return;
+      }
+      w->bytes = bytes;
+    } else if (hipStreamWaitEvent(s, w->done, 0) != hipSuccess) {
+      err = "hipStreamWaitEvent failed";
+      return;
+    }
+    p = w->p;
+  }
+  // Marks the end of this call's use (after its last enqueued access).
+  ~WorkspaceLease() {
+    if (p) (void)hipEventRecord(w->done, s);
+  }
+};
+
+// Bytes of scratch a batch of transforms needs: the split-layout copy (main
+// block + Nyquist planes).
+template <typename R> size_t r2c_scratch_bytes(int nx, int ny, int nz, int batch) {
+  return sizeof(R) * (size_t)nx * ny * nz * batch + sizeof(c2<R>) * (size_t)ny * nz * batch;
+}
+
+// The forward transform with its scratch at ws.
 template <typename R>
-int r2c_dev(int nx, int ny, int nz, int batch, const void* in, void* out, int layout, double scale,
-            hipStream_t s) {
-  const size_t P = (size_t)nx * ny * nz, plane = (size_t)ny * nz;
-  char* ws = (char*)workspace(sizeof(R) * P * batch + sizeof(c2<R>) * plane * batch);
-  if (!ws) return fail(PF_ENOMEM, "workspace allocation failed");
+int r2c_core(int nx, int ny, int nz, int batch, const void* in, void* out, int layout, double scale,
+             void* wsp, hipStream_t s) {
+  const size_t P = (size_t)nx * ny * nz;
+  char* ws = (char*)wsp;
   c2<R>* main = (c2<R>*)ws;
   c2<R>* nyq = (c2<R>*)(ws + sizeof(R) * P * batch);
   PF_HIP(hipMemcpyAsync(main, in, sizeof(R) * P * batch, hipMemcpyDeviceToDevice, s));
@@ -422,10 +538,19 @@ int r2c_dev(int nx, int ny, int nz, int batch, const void* in, void* out, int la
 }
 
 template <typename R>
+int r2c_dev(int nx, int ny, int nz, int batch, const void* in, void* out, int layout, double scale,
+            hipStream_t s) {
+  WorkspaceLease lease(r2c_scratch_bytes<R>(nx, ny, nz, batch), s);
+  if (!lease.p) return fail(PF_ENOMEM, lease.err);
+  return r2c_core<R>(nx, ny, nz, batch, in, out, layout, scale, lease.p, s);
+}
+
+template <typename R>
 int c2r_dev(int nx, int ny, int nz, int batch, const void* in, void* out, int layout, hipStream_t s) {
   const size_t P = (size_t)nx * ny * nz, plane = (size_t)ny * nz;
-  char* ws = (char*)workspace(sizeof(R) * P * batch + sizeof(c2<R>) * plane * batch);
-  if (!ws) return fail(PF_ENOMEM, "workspace allocation failed");
+  WorkspaceLease lease(sizeof(R) * P * batch + sizeof(c2<R>) * plane * batch, s);
+  if (!lease.p) return fail(PF_ENOMEM, lease.err);
+  char* ws = (char*)lease.p;
   c2<R>* main = (c2<R>*)ws;
   c2<R>* nyq = (c2<R>*)(ws + sizeof(R) * P * batch);
   int rc = unpack_spec<R>((const c2<R>*)in, main, nyq, nx, ny, nz, batch, layout, s);
@@ -462,6 +587,43 @@ extern "C" int pf_fft_c2r(int nx, int ny, int nz, int batch, int prec, const voi
   return fail(PF_EINVAL, "bad precision");
 }
 
+// Device buffers of pf_f_trans / pf_inv_trans, per device, grown as needed and
+// kept until the process ends.  These calls are synchronous: the lock is held
+// for the whole call, so calls from several threads take turns.
+namespace {
+struct HostCallBuffers {
+  std::mutex mu;
+  void* in = nullptr;
+  void* out = nullptr;
+  size_t in_bytes = 0, out_bytes = 0;
+};
+std::mutex g_hcb_mu;
+std::map<int, std::unique_ptr<HostCallBuffers>> g_hcb;
+
+HostCallBuffers* host_call_buffers() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_hcb_mu);
+  auto& b = g_hcb[dev];
+  if (!b) b.reset(new HostCallBuffers);
+  return b.get();
+}
+
+int grow(void** p, size_t* have, size_t want) {
+  if (*have >= want) return 0;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *have = 0;
+  if (hipMalloc(p, want) != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(PF_ENOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+  }
+  *have = want;
+  return 0;
+}
+}  // namespace
+
 // Host-buffer drop-ins for f_trans (src/f_trans.f90:1-15) and inv_trans
 // (src/inv_trans.f90:1-13): FP64, the spectrum in the reference's Z-pencil order.
 extern "C" int pf_f_trans(const double* in, double* out, int nx, int ny, int nz) {
@@ -469,18 +631,15 @@ extern "C" int pf_f_trans(const double* in, double* out, int nx, int ny, int nz)
   if (rc) return rc;
   if (!in || !out) return fail(PF_EINVAL, "null buffer");
   const size_t P = (size_t)nx * ny * nz, H = (size_t)(nx / 2 + 1) * ny * nz;
-  void *din = nullptr, *dout = nullptr;
-  PF_HIP(hipMalloc(&din, sizeof(double) * P));
-  if (hipMalloc(&dout, sizeof(double) * 2 * H) != hipSuccess) {
-    (void)hipFree(din);
-    return fail(PF_ENOMEM, "device allocation failed");
-  }
-  rc = to_device(din, in, sizeof(double) * P, nullptr);
+  HostCallBuffers* b = host_call_buffers();
+  if (!b) return fail(PF_EHIP, "no current device");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if ((rc = grow(&b->in, &b->in_bytes, sizeof(double) * P))) return rc;
+  if ((rc = grow(&b->out, &b->out_bytes, sizeof(double) * 2 * H))) return rc;
+  rc = to_device(b->in, in, sizeof(double) * P, nullptr);
   const double scale = 1.0 / (double)(float)((long)nx * ny * nz);
-  if (!rc) rc = pf_fft_r2c(nx, ny, nz, 1, PF_PREC_FP64, din, dout, PF_LAYOUT_ZFAST, scale, nullptr);
-  if (!rc) rc = to_host(out, dout, sizeof(double) * 2 * H, nullptr);
-  (void)hipFree(din);
-  (void)hipFree(dout);
+  if (!rc) rc = pf_fft_r2c(nx, ny, nz, 1, PF_PREC_FP64, b->in, b->out, PF_LAYOUT_ZFAST, scale, nullptr);
+  if (!rc) rc = to_host(out, b->out, sizeof(double) * 2 * H, nullptr);
   return rc;
 }
 
@@ -489,16 +648,13 @@ extern "C" int pf_inv_trans(const double* in, double* out, int nx, int ny, int n
   if (rc) return rc;
   if (!in || !out) return fail(PF_EINVAL, "null buffer");
   const size_t P = (size_t)nx * ny * nz, H = (size_t)(nx / 2 + 1) * ny * nz;
-  void *din = nullptr, *dout = nullptr;
-  PF_HIP(hipMalloc(&din, sizeof(double) * 2 * H));
-  if (hipMalloc(&dout, sizeof(double) * P) != hipSuccess) {
-    (void)hipFree(din);
-    return fail(PF_ENOMEM, "device allocation failed");
-  }
-  rc = to_device(din, in, sizeof(double) * 2 * H, nullptr);
-  if (!rc) rc = pf_fft_c2r(nx, ny, nz, 1, PF_PREC_FP64, din, dout, PF_LAYOUT_ZFAST, nullptr);
-  if (!rc) rc = to_host(out, dout, sizeof(double) * P, nullptr);
-  (void)hipFree(din);
-  (void)hipFree(dout);
+  HostCallBuffers* b = host_call_buffers();
+  if (!b) return fail(PF_EHIP, "no current device");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if ((rc = grow(&b->in, &b->in_bytes, sizeof(double) * 2 * H))) return rc;
+  if ((rc = grow(&b->out, &b->out_bytes, sizeof(double) * P))) return rc;
+  rc = to_device(b->in, in, sizeof(double) * 2 * H, nullptr);
+  if (!rc) rc = pf_fft_c2r(nx, ny, nz, 1, PF_PREC_FP64, b->in, b->out, PF_LAYOUT_ZFAST, nullptr);
+  if (!rc) rc = to_host(out, b->out, sizeof(double) * P, nullptr);
   return rc;
 }

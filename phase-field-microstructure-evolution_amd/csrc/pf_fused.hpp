@@ -20,6 +20,19 @@ struct UPar {
   double dt, kphi, kc, D, L, Dgb, am, ap, cm, cp, s1g, s2g, invP;
 };
 
+// One simulation of a batched ensemble (a device table of these drives the
+// BATCH forms of the step kernels: one launch per stage covers every member of
+// one grid shape): its carried spectrum S, work bank W, FP64 k = 0 modes and
+// parameters.
+template <typename R> struct Member {
+  c2<R>* Wm;
+  c2<R>* Wn;
+  c2<R>* Sm;
+  c2<R>* Sn;
+  double* dc64;
+  UPar u;
+};
+
 // Deterministic block reductions for any blockDim (multiple of 64).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -195,10 +208,19 @@ constexpr int kXlineRows = 2;
 
 __host__ __device__ constexpr int xline_row_pitch(int M) { return M | 1; }
 
-template <typename R, int NX, int LINES, bool XONLY>
+// BATCH: ngroups counts the line groups of every member in mt (member m owns
+// groups m*gpm .. m*gpm + gpm - 1, gpm = plane / LINES), each taking its banks
+// and parameters from its entry; wm, wn, u are unused; no diagnostics.
+template <typename R> __device__ __forceinline__ const UPar& member_u(const UPar& u, const Member<R>* mt, long m,
+                                                                    bool batch) {
+  return batch ? mt[m].u : u;
+}
+
+template <typename R, int NX, int LINES, bool XONLY, bool BATCH = false>
 __global__ void __launch_bounds__(kXlineMaxThreads<R>) __attribute__((amdgpu_waves_per_eu(kXlineWaves<R>)))
 k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __restrict__ mobd,
-        const c2<R>* __restrict__ tw, double* __restrict__ part, long ngroups, int diag, int mob) {
+        const c2<R>* __restrict__ tw, double* __restrict__ part, long ngroups, int diag, int mob,
+        const Member<R>* __restrict__ mt) {
   constexpr int M = NX / 2, T = plan_T(M), E = plan_E(M);
   constexpr int MP = xline_row_pitch(M);
   static_assert(last_pass_in_place<M>(), "the write-backs after the x transforms rely on it");
@@ -231,7 +253,17 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const long chunk = (ngroups + gridDim.x - 1) / gridDim.x;
   const long lg_end = ((lb + 1) * chunk < ngroups) ? (lb + 1) * chunk : ngroups;
   for (long lg = lb * chunk; lg < lg_end; ++lg) {
-    const long L0 = lg * LINES;
+    long L0 = lg * LINES, m = 0;
+    c2<R>* wmm = wm;
+    c2<R>* wnn = wn;
+    if constexpr (BATCH) {
+      const long gpm = g.plane / LINES;
+      m = lg / gpm;
+      L0 = (lg - m * gpm) * LINES;
+      wmm = mt[m].Wm;
+      wnn = mt[m].Wn;
+    }
+    const UPar& uu = member_u<R>(u, mt, m, BATCH);
     // 1. half lines of every field -> LDS rows: wave w stages rows w, w + nw, ...
     //    (coalesced along k), kXlineRows rows per batch in flight
     for (int jb0 = wv; jb0 < njobs; jb0 += kXlineRows * nw) {
@@ -241,7 +273,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
         const int jb = jb0 + r * nw;
         if (jb < njobs) {
           const int f = jb / LINES, l = jb - f * LINES;
-          const c2<R>* src = wm + (long)f * g.hmain + (L0 + l) * M;
+          const c2<R>* src = wmm + (long)f * g.hmain + (L0 + l) * M;
 #pragma unroll
           for (int kk = 0; kk < KPL; ++kk)
             if (M % 64 == 0 || lane + 64 * kk < M) tmp[r][kk] = ld_nt<R>(src + lane + 64 * kk);
@@ -259,7 +291,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     }
     for (int i = tid; i < njobs; i += nt) {
       const int f = i / LINES, l = i - f * LINES;
-      shn[i] = wn[(long)f * g.plane + L0 + l].x;
+      shn[i] = wnn[(long)f * g.plane + L0 + l].x;
     }
     __syncthreads();
     // 2. C2R pre-processing straight into the transform's input registers
@@ -295,8 +327,8 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
         c2<R>* vv = sh + (long)l * MP + k;
         const long gp = (L0 + l) * NX + 2 * k;  // point index within a field
         R mo[2];
-        if (diag) nl_points<R, 2, true>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
-        else nl_points<R, 2, false>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+        if (diag) nl_points<R, 2, true>(vv, fs, g.var, uu, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+        else nl_points<R, 2, false>(vv, fs, g.var, uu, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
         if (mob) {
           mobd[gp] = mo[0];
           mobd[gp + 1] = mo[1];
@@ -313,8 +345,8 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
           vv[0] = vv[0] * mobd[gp];
         } else {
           R mo[1];
-          if (diag) nl_points<R, 1, true>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
-          else nl_points<R, 1, false>(vv, fs, g.var, u, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+          if (diag) nl_points<R, 1, true>(vv, fs, g.var, uu, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
+          else nl_points<R, 1, false>(vv, fs, g.var, uu, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
           if (mob) mobd[gp] = mo[0];
         }
       }
@@ -333,7 +365,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     const R hs = R(0.5) * scale;
     for (int jb = wv; jb < njobs; jb += nw) {
       const int f = jb / LINES, l = jb - f * LINES;
-      c2<R>* dst = wm + (long)f * g.hmain + (L0 + l) * M;
+      c2<R>* dst = wmm + (long)f * g.hmain + (L0 + l) * M;
       const c2<R>* rw = sh + jb * MP;
 #pragma unroll
       for (int kk = 0; kk < KPL; ++kk) {
@@ -344,7 +376,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     for (int i = tid; i < njobs; i += nt) {
       const int f = i / LINES, l = i - f * LINES;
       const c2<R> z0 = sh[i * MP];
-      wn[(long)f * g.plane + L0 + l] = mk<R>((z0.x - z0.y) * scale, R(0));
+      wnn[(long)f * g.plane + L0 + l] = mk<R>((z0.x - z0.y) * scale, R(0));
     }
     __syncthreads();
   }
@@ -363,10 +395,12 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
 // k_ypass: y transform of the main block (column tiles of W = M, panels (z, f))
 // and of the Nyquist planes (contiguous lines of length NY), one launch.
 // ---------------------------------------------------------------------------
-template <typename R, int NY, bool INV>
+// BATCH: the W banks of every member in mt, ppm panels (and Nyquist lines) per
+// member (fields x nz); main and nyq are unused.
+template <typename R, int NY, bool INV, bool BATCH = false>
 __global__ void __launch_bounds__(col_tile(NY, sizeof(c2<R>)) * plan_T(NY))
 k_ypass(c2<R>* __restrict__ main, c2<R>* __restrict__ nyq, long M, long npanels, long nlines,
-        const c2<R>* __restrict__ tw) {
+        const c2<R>* __restrict__ tw, const Member<R>* __restrict__ mt, long ppm) {
   constexpr int T = plan_T(NY), E = plan_E(NY);
   constexpr int TX = col_tile(NY, sizeof(c2<R>));
   __shared__ c2<R> sh[(plan_np(NY) > 1) ? NY * TX : 1];
@@ -375,12 +409,19 @@ k_ypass(c2<R>* __restrict__ main, c2<R>* __restrict__ nyq, long M, long npanels,
   const long b = xcd_block(blockIdx.x, gridDim.x);
   c2<R> v[E];
   if (b < nmain) {
-    const long pnl = b / tpp;
-    const long col0 = (b - pnl * tpp) * TX;
+    const long pl = b / tpp;
+    long pnl = pl;
+    c2<R>* mb = main;
+    if constexpr (BATCH) {
+      const long m = pl / ppm;
+      pnl = pl - m * ppm;
+      mb = mt[m].Wm;
+    }
+    const long col0 = (b - pl * tpp) * TX;
     const int ncol = (int)((M - col0) < TX ? (M - col0) : TX);
     const int c = threadIdx.x % TX, t = threadIdx.x / TX;
     const bool act = c < ncol;
-    c2<R>* gp = main + pnl * NY * M + col0 + c;
+    c2<R>* gp = mb + pnl * NY * M + col0 + c;
 #pragma unroll
     for (int s = 0; s < kInSlots<NY>; ++s)
       if (act && rfft_in_valid<NY>(t, s)) v[s] = ld_nt<R>(gp + (long)rfft_in_idx<NY>(t, s) * M);
@@ -392,7 +433,15 @@ k_ypass(c2<R>* __restrict__ main, c2<R>* __restrict__ nyq, long M, long npanels,
     const int l = threadIdx.x / T, t = threadIdx.x % T;
     const long line = (b - nmain) * TX + l;
     const bool act = line < nlines;
-    c2<R>* gp = nyq + (act ? line : 0) * NY;
+    long gl = line;
+    c2<R>* nb = nyq;
+    if constexpr (BATCH) {
+      const long ln = act ? line : 0;
+      const long m = ln / ppm;
+      gl = ln - m * ppm;
+      nb = mt[m].Wn;
+    }
+    c2<R>* gp = nb + (act ? gl : 0) * NY;
 #pragma unroll
     for (int s = 0; s < kInSlots<NY>; ++s)
       if (act && rfft_in_valid<NY>(t, s)) v[s] = ld_nt<R>(gp + rfft_in_idx<NY>(t, s));
@@ -432,12 +481,24 @@ __device__ __forceinline__ double recip_fast(double x) {
 // (158 VGPRs for the plain update, 168 with the Parseval sums, no spills), FP64 2.
 template <typename R, bool PARSEVAL> constexpr int kZlineWaves = sizeof(R) == 4 ? 3 : 2;
 
-template <typename R, int NZ, int ROLE, bool DC64, bool PARSEVAL>
+// The update coefficients of one parameter set, FP64.
+struct ZCo {
+  double Dk, Ddt, Lkd, Ldt, dt, kc, kphi;
+};
+__device__ __forceinline__ ZCo zco(const UPar& u) {
+  return ZCo{(u.D * u.dt) * u.kc, u.D * u.dt, (u.L * u.kphi) * u.dt, u.L * u.dt, u.dt, u.kc, u.kphi};
+}
+
+// BATCH (ROLE_REG only): the fields f0 .. f0 + nf - 1 run over every member in
+// mt, g.F per member, each with its own S, W, k = 0 modes and parameters; S, W
+// and dc64 are unused.
+template <typename R, int NZ, int ROLE, bool DC64, bool PARSEVAL, bool BATCH = false>
 __global__ void __launch_bounds__(col_tile(NZ, sizeof(c2<R>)) * plan_T(NZ))
 __attribute__((amdgpu_waves_per_eu(kZlineWaves<R, PARSEVAL>)))
 k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
         double* __restrict__ dc64, double* __restrict__ part, const c2<R>* __restrict__ tw, int f0, int nf,
-        long tm, long tn) {
+        long tm, long tn, const Member<R>* __restrict__ mt) {
+  static_assert(!BATCH || ROLE == ROLE_REG, "batched members step without the mobility chain");
   constexpr int TX = col_tile(NZ, sizeof(c2<R>));
   constexpr int NI = kInSlots<NZ>, NO = kOutSlots<NZ>, E = plan_E(NZ);
   __shared__ c2<R> sh[(plan_np(NZ) > 1) ? NZ * TX : 1];
@@ -445,15 +506,25 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
   const long tpf = tm + tn;
   const long total = (long)nf * tpf;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const double Dk = (u.D * u.dt) * u.kc, Ddt = u.D * u.dt, Lkd = (u.L * u.kphi) * u.dt;
-  const double Ldt = u.L * u.dt, dtQ = u.dt, kcQ = u.kc;
+  const ZCo z0c = zco(u);
   double acc = 0.0;
   c2<R> v[E];
   c2<R> sv[NO];
   c2<R> xv[(ROLE == ROLE_CMOB) ? NO : 1];
   const long tile0 = xcd_runs(blockIdx.x, gridDim.x, ((long)g.M + TX - 1) / TX);
   for (long tile = tile0; tile < total; tile += gridDim.x) {
-    const int f = f0 + (int)(tile / tpf);
+    int f = f0 + (int)(tile / tpf);
+    Bank<R> Sb = S, Wb = W;
+    double* dcb = dc64;
+    ZCo zc = z0c;
+    if constexpr (BATCH) {
+      const long m = f / g.F;
+      f -= (int)(m * g.F);
+      Sb = Bank<R>{mt[m].Sm, mt[m].Sn};
+      Wb = Bank<R>{mt[m].Wm, mt[m].Wn};
+      dcb = mt[m].dc64;
+      zc = zco(mt[m].u);
+    }
     const long tt = tile % tpf;
     const bool isn = tt >= tm;
     const long Wd = isn ? (long)g.ny : (long)g.M * g.ny;
@@ -462,8 +533,8 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
     const bool act = c < ncol;
     const long off = (isn ? (long)f * g.plane : (long)f * g.hmain) + col0 + c;
     const long off0 = col0 + c;  // single-field banks X, Y (c-only roles)
-    c2<R>* const wb = (isn ? W.nyq : W.main) + off;
-    c2<R>* const sb = (isn ? S.nyq : S.main) + off;
+    c2<R>* const wb = (isn ? Wb.nyq : Wb.main) + off;
+    c2<R>* const sb = (isn ? Sb.nyq : Sb.main) + off;
     c2<R>* const xb = (isn ? X.nyq : X.main) + off0;
     const c2<R>* const ib = (ROLE == ROLE_CMOB) ? (isn ? Y.nyq : Y.main) + off0 : wb;
     // 1. all loads of the tile at once: transform input, spectrum (and f^) at the output rows
@@ -490,7 +561,7 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
       ktz.az += zo;
       ktz.azp += zo;
     }
-    const double kap = (f == 0) ? u.kc : u.kphi;
+    const double kap = (f == 0) ? zc.kc : zc.kphi;
     // 2. forward z, update in registers
     rfft<R, NZ, false, TX>(v, t, act, sh + c, tw);
 #pragma unroll
@@ -504,18 +575,18 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
       if constexpr (ROLE == ROLE_GHAT) {
         const c2<R> ch = sv[s];
         xb[ga] = w;  // f^
-        sn = mk<R>((R)(-(k.s1 * w.y) + kcQ * ch.y * k.k3), (R)(k.s1 * w.x - kcQ * ch.x * k.k3));
+        sn = mk<R>((R)(-(k.s1 * w.y) + zc.kc * ch.y * k.k3), (R)(k.s1 * w.x - zc.kc * ch.x * k.k3));
       } else {
         const c2<R> s0 = sv[s];
         // A = 1/den multiplies the whole state each step; B (and Cc) the step's
         // increment
-        const double den = (f == 0) ? 1.0 + Dk * k.k4 : 1.0 - Lkd * k.k2;
+        const double den = (f == 0) ? 1.0 + zc.Dk * k.k4 : 1.0 - zc.Lkd * k.k2;
         if constexpr (sizeof(R) == 8) {
           const double A = 1.0 / den;
-          const double B = (f == 0) ? (Ddt * k.k2) * A : -Ldt * A;
+          const double B = (f == 0) ? (zc.Ddt * k.k2) * A : -zc.Ldt * A;
           if constexpr (ROLE == ROLE_CMOB) {
             const c2<R> fh = xv[s];
-            const double Cc = (dtQ * k.s1) * A;
+            const double Cc = (zc.dt * k.s1) * A;
             sn = mk<R>(A * s0.x + B * fh.x - Cc * w.y, A * s0.y + B * fh.y + Cc * w.x);
           } else {
             sn = mk<R>(A * s0.x + B * w.x, A * s0.y + B * w.y);
@@ -525,19 +596,19 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
           // the only per-step error on the state is the rounding of the products
           const double A = recip_fast(den);
           const float Ah = (float)A, Al = (float)(A - (double)Ah);
-          const float B = (float)((f == 0) ? (Ddt * k.k2) * A : -Ldt * A);
+          const float B = (float)((f == 0) ? (zc.Ddt * k.k2) * A : -zc.Ldt * A);
           float ix = B * (float)w.x, iy = B * (float)w.y;
           if constexpr (ROLE == ROLE_CMOB) {
             const c2<R> fh = xv[s];
-            const float Cc = (float)((dtQ * k.s1) * A);
+            const float Cc = (float)((zc.dt * k.s1) * A);
             ix = B * fh.x - Cc * w.y;
             iy = B * fh.y + Cc * w.x;
           }
           sn = mk<R>(fmaf(Ah, s0.x, fmaf(Al, s0.x, ix)), fmaf(Ah, s0.y, fmaf(Al, s0.y, iy)));
         }
         if (DC64 && kx == 0 && ky == 0 && e == 0) {
-          if (f != 0) dc64[f] = dc64[f] - (u.L * u.dt) * (double)w.x;
-          sn = mk<R>((R)dc64[f], R(0));
+          if (f != 0) dcb[f] = dcb[f] - zc.Ldt * (double)w.x;
+          sn = mk<R>((R)dcb[f], R(0));
         }
         st_nt<R>(sb + ga, sn);
 

@@ -58,6 +58,50 @@ __global__ void k_gather_state(const R* __restrict__ src, double* __restrict__ p
   }
 }
 
+// Voronoi polycrystal built on the device (the synthetic initial condition of
+// pf_ic_voronoi, same draw order): point t = (i, j, k), x fastest, belongs to
+// the nearest seed by periodic minimum-image distance, the lowest index on ties;
+// eta_{(s mod var)+1} = 1 there and the other eta 0; c is the host's draw.  The
+// distances use the host generator's FP64 operations in its order, with no
+// contraction into FMAs, so every owner equals the host's.  counts[v] gains the
+// number of points owned by field v + 1 (exact: the field means).
+__device__ __forceinline__ double vor_wrap(double d, int n) {
+  d = fabs(d);
+  const double alt = (double)n - d;
+  return alt < d ? alt : d;
+}
+template <typename R>
+__global__ void __launch_bounds__(NB) k_voronoi_state(Geo g, const double* __restrict__ seeds, int nseeds,
+                                                      const double* __restrict__ con, R* __restrict__ dst,
+                                                      unsigned long long* __restrict__ counts) {
+  extern __shared__ unsigned int hist[];  // g.var counters
+  for (int v = threadIdx.x; v < g.var; v += NB) hist[v] = 0u;
+  __syncthreads();
+  for (long t = (long)blockIdx.x * NB + threadIdx.x; t < g.P; t += (long)gridDim.x * NB) {
+    const int i = (int)(t % g.nx), j = (int)((t / g.nx) % g.ny), k = (int)(t / ((long)g.nx * g.ny));
+    int best = 0;
+    double bd = 0.0;
+    for (int s = 0; s < nseeds; ++s) {
+#pragma clang fp contract(off)
+      const double dx = vor_wrap((double)i - seeds[3 * s], g.nx);
+      const double dy = vor_wrap((double)j - seeds[3 * s + 1], g.ny);
+      const double dz = vor_wrap((double)k - seeds[3 * s + 2], g.nz);
+      const double d2 = (dx * dx + dy * dy) + dz * dz;
+      if (s == 0 || d2 < bd) {
+        bd = d2;
+        best = s;
+      }
+    }
+    const int own = best % g.var;
+    dst[t] = (R)con[t];
+    for (int v = 0; v < g.var; ++v) dst[(long)(v + 1) * g.P + t] = (v == own) ? R(1) : R(0);
+    atomicAdd(&hist[own], 1u);
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < g.var; v += NB)
+    if (hist[v]) atomicAdd(&counts[v], (unsigned long long)hist[v]);
+}
+
 // ---------------------------------------------------------------------------
 // Loops 5-10, faithful form (src/var_diff.f90:685-905), FP64, O(var^2).
 // Reads the real state bank (c, eta_i); writes df_dc, df_dphi_i, mob_c.
@@ -456,6 +500,11 @@ __global__ void k_set_int(int* __restrict__ p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+// p[0 .. n) = 0 (a kernel rather than a memset, so it may sit in a captured graph)
+__global__ void k_fill_zero(double* __restrict__ p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
 }  // namespace pf
 
 // ===========================================================================
@@ -779,7 +828,7 @@ int launch_xline_n(pf_handle* h, c2<R>* wm, c2<R>* wn, R* mobd, double* part, lo
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shb) != hipSuccess)
     return fail(PF_EHIP, "x-line kernel: " + std::to_string(shb) + " bytes of LDS refused");
   hipLaunchKernelGGL((k_xline<R, NX, LINES, XONLY>), dim3((unsigned)grid), dim3((unsigned)nt), shb, h->s, g, h->u,
-                     wm, wn, mobd, tw, part, ngroups, (int)diag, (int)mob);
+                     wm, wn, mobd, tw, part, ngroups, (int)diag, (int)mob, (const Member<R>*)nullptr);
   if (grid_out) *grid_out = grid;
   return 0;
 }
@@ -846,7 +895,7 @@ int launch_ypass_fused(pf_handle* h, c2<R>* main, c2<R>* nyq, int nf) {
     const c2<R>* tw = twiddle_table<R>(N);                                                      \
     if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");                           \
     hipLaunchKernelGGL((k_ypass<R, N, INV>), dim3((unsigned)grid), dim3(TX * plan_T(N)), 0, \
-                       h->s, main, nyq, (long)g.M, npanels, nlines, tw);                        \
+                       h->s, main, nyq, (long)g.M, npanels, nlines, tw, (const Member<R>*)nullptr, 0L); \
     return 0;                                                                                   \
   }
     PF_GRID_SIZES(PF_CASE)
@@ -890,8 +939,98 @@ int launch_zline(pf_handle* h, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y, doubl
     if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");                                \
     hipLaunchKernelGGL((k_zline<R, N, ROLE, DC64, PARSEVAL>), dim3((unsigned)grid),                  \
                        dim3(TX * plan_T(N)), 0, h->s, g, h->u, h->kt, S, W, X, Y, h->dc64, part, tw,      \
-                       f0, nf, tm, tn);                                                              \
+                       f0, nf, tm, tn, (const Member<R>*)nullptr);                                   \
     if (grid_out) *grid_out = grid;                                                                  \
+    return 0;                                                                                        \
+  }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported nz");
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched ensemble: one launch per stage over every member of a group (same
+// grid and var, FP32-mixed form, constant mobility), each member's banks and
+// parameters read from the device table mt.  The kernels and their per-point
+// arithmetic are those of a single member's step, so each member's result is
+// bit-identical to stepping it alone.
+// ---------------------------------------------------------------------------
+int xline_batch(const Geo& g, const UPar& u, const Member<float>* mt, long nmem, hipStream_t s) {
+  long nt = 0;
+  size_t shb = 0;
+  switch (g.nx) {
+#define PF_TRY(N, L)                                                                                   \
+  if (xline_fits<float, N, L, false>(g, &nt, &shb)) {                                                  \
+    const long ngroups = nmem * (g.plane / L);                                                         \
+    long grid = std::min<long>(ngroups, kMaxGrid);                                                     \
+    if (grid >= 8) grid -= grid % 8;                                                                   \
+    const c2<float>* tw = twiddle_table<float>(N);                                                     \
+    if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");                                      \
+    if (shb > 65536 &&                                                                                 \
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_xline<float, N, L, false, true>),         \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shb) != hipSuccess)       \
+      return fail(PF_EHIP, "x-line kernel: " + std::to_string(shb) + " bytes of LDS refused");         \
+    hipLaunchKernelGGL((k_xline<float, N, L, false, true>), dim3((unsigned)grid), dim3((unsigned)nt), shb, \
+                       s, g, u, (c2<float>*)nullptr, (c2<float>*)nullptr, (float*)nullptr, tw,         \
+                       (double*)nullptr, ngroups, 0, 0, mt);                                            \
+    return 0;                                                                                          \
+  }
+#define PF_CASE(N)                                                                                     \
+  case N: {                                                                                            \
+    const long T = plan_T(N / 2);                                                                      \
+    int want = 1;                                                                                      \
+    while (want < 8 && g.F * want * T < kXlineMinThreads) want *= 2;                                   \
+    if (want >= 8) { PF_TRY(N, 8) }                                                                    \
+    if (want >= 4) { PF_TRY(N, 4) }                                                                    \
+    if (want >= 2) { PF_TRY(N, 2) }                                                                    \
+    PF_TRY(N, 1)                                                                                       \
+    return fail(PF_ENOTIMPL, "x-line kernel: " + std::to_string(g.F) +                                \
+                                 " fields of length " #N " exceed one block's threads or LDS");        \
+  }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+#undef PF_TRY
+    default: return fail(PF_EINVAL, "unsupported nx");
+  }
+}
+
+template <bool INV> int ypass_batch(const Geo& g, const Member<float>* mt, long nmem, hipStream_t s) {
+  switch (g.ny) {
+#define PF_CASE(N)                                                                                    \
+  case N: {                                                                                           \
+    constexpr int TX = col_tile(N, sizeof(c2<float>));                                                \
+    const long ppm = (long)g.nz * g.F, npanels = nmem * ppm, nlines = npanels;                        \
+    const long tpp = ((long)g.M + TX - 1) / TX;                                                       \
+    const long grid = (tpp * npanels + (nlines + TX - 1) / TX + 7) / 8 * 8;                           \
+    const c2<float>* tw = twiddle_table<float>(N);                                                    \
+    if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");                                     \
+    hipLaunchKernelGGL((k_ypass<float, N, INV, true>), dim3((unsigned)grid), dim3(TX * plan_T(N)), 0, s, \
+                       (c2<float>*)nullptr, (c2<float>*)nullptr, (long)g.M, npanels, nlines, tw, mt, ppm); \
+    return 0;                                                                                         \
+  }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return fail(PF_EINVAL, "unsupported ny");
+  }
+}
+
+int zline_batch(const Geo& g, const UPar& u, const KTab& kt, const Member<float>* mt, long nmem, hipStream_t s) {
+  const Bank<float> none{nullptr, nullptr};
+  switch (g.nz) {
+#define PF_CASE(N)                                                                                   \
+  case N: {                                                                                          \
+    constexpr int TX = col_tile(N, sizeof(c2<float>));                                               \
+    const long tm = ((long)g.M * g.ny + TX - 1) / TX, tn = ((long)g.ny + TX - 1) / TX;               \
+    const long nf = nmem * g.F;                                                                      \
+    long grid = std::min<long>((tm + tn) * nf, kZlineMaxGrid);                                       \
+    const long run = 8 * (((long)g.M + TX - 1) / TX);                                                \
+    if ((grid + run - 1) / run * run <= kZlineMaxGrid) grid = (grid + run - 1) / run * run;         \
+    const c2<float>* tw = twiddle_table<float>(N);                                                   \
+    if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");                                    \
+    hipLaunchKernelGGL((k_zline<float, N, ROLE_REG, true, false, true>), dim3((unsigned)grid),       \
+                       dim3(TX * plan_T(N)), 0, s, g, u, kt, none, none, none, none, (double*)nullptr, \
+                       (double*)nullptr, tw, 0, (int)nf, tm, tn, mt);                                \
     return 0;                                                                                        \
   }
     PF_GRID_SIZES(PF_CASE)
@@ -1026,25 +1165,31 @@ std::vector<double> host_means(const double* phi, const double* con, long P, int
   return dc;
 }
 
-template <typename R> int put_reduced(pf_handle* h, const double* phi, const double* con) {
+// Reduced forms, once W holds the real state (SoA): S <- F(W) with f_trans
+// normalisation, W <- x-half lines, dc64 <- the FP64 field means dc.
+template <typename R> int finish_put_reduced(pf_handle* h, const std::vector<double>& dc) {
   const Geo& g = h->g;
-  // W <- real state; S <- F(W) with f_trans normalisation
-  hipLaunchKernelGGL((k_scatter_state<R>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s, h->staging,
-                     h->staging + (size_t)g.var * g.P, (R*)h->W_main, g.P, g.var);
   PF_HIP(hipMemcpyAsync(h->S_main, h->W_main, sizeof(R) * (size_t)g.P * g.F, hipMemcpyDeviceToDevice, h->s));
   int rc = fwd3d<R>((c2<R>*)h->S_main, (c2<R>*)h->S_nyq, g.nx, g.ny, g.nz, g.F, (R)(1.0 / (double)(float)g.P), h->s);
   if (rc) return rc;
   // W <- x-half lines: (1/nx) R2C along x of the real state == z,y-inverse of S
   rc = x_r2c<R>(g.nx, (c2<R>*)h->W_main, (c2<R>*)h->W_nyq, (long)g.ny * g.nz * g.F, (R)(1.0 / (double)g.nx), h->s);
   if (rc) return rc;
+  PF_HIP(hipMemcpyAsync(h->dc64, dc.data(), sizeof(double) * g.F, hipMemcpyHostToDevice, h->s));
+  PF_HIP(hipStreamSynchronize(h->s));
+  return 0;
+}
+
+template <typename R> int put_reduced(pf_handle* h, const double* phi, const double* con) {
+  const Geo& g = h->g;
+  // W <- real state
+  hipLaunchKernelGGL((k_scatter_state<R>), dim3(grid_for(g.P, NB)), dim3(NB), 0, h->s, h->staging,
+                     h->staging + (size_t)g.var * g.P, (R*)h->W_main, g.P, g.var);
   // FP64 means (k = 0 modes), Neumaier-compensated: one pass over the host
   // arrays in kMeanChunks fixed chunks (threads take whole chunks), chunk
   // partials combined in chunk order, so the result does not depend on the
   // thread count
-  std::vector<double> dc = host_means(phi, con, g.P, g.var);
-  PF_HIP(hipMemcpyAsync(h->dc64, dc.data(), sizeof(double) * g.F, hipMemcpyHostToDevice, h->s));
-  PF_HIP(hipStreamSynchronize(h->s));
-  return 0;
+  return finish_put_reduced<R>(h, host_means(phi, con, g.P, g.var));
 }
 
 }  // namespace
@@ -1163,6 +1308,68 @@ extern "C" int pf_put_state(pf_handle* h, const double* phi, const double* con) 
   return put_reduced<float>(h, phi, con);
 }
 
+// Largest seed count of the device-built Voronoi state (seeds are read from
+// global memory by every thread; this bounds the per-point search).
+constexpr int kMaxVoronoiSeeds = 1 << 16;
+
+extern "C" int pf_put_voronoi(pf_handle* h, int nseeds, int iseed, double c0, double c_amp) {
+  if (!h) return fail(PF_EINVAL, "null handle");
+  if (nseeds < 1 || nseeds > kMaxVoronoiSeeds)
+    return fail(PF_EINVAL, "nseeds must be in [1, " + std::to_string(kMaxVoronoiSeeds) + "]");
+  DeviceGuard dg(h->dev);
+  const Geo& g = h->g;
+  // the draws of pf_ic_voronoi: 3 per seed (x, y, z), then one c per point
+  pf_ran2_state st;
+  pf_ran2_init(&st);
+  int idum = iseed;
+  std::vector<double> seeds(3 * (size_t)nseeds);
+  for (int s = 0; s < nseeds; ++s) {
+    seeds[3 * s] = pf_ran2(&st, &idum) * g.nx;
+    seeds[3 * s + 1] = pf_ran2(&st, &idum) * g.ny;
+    seeds[3 * s + 2] = pf_ran2(&st, &idum) * g.nz;
+  }
+  std::vector<double> con((size_t)g.P);
+  for (long t = 0; t < g.P; ++t) con[t] = c0 + c_amp * (2.0 * pf_ran2(&st, &idum) - 1.0);
+  // staging: c at its usual place (after the var*P phi values), seeds and the
+  // owner counts in the (unused here) phi area
+  double* dcon = h->staging + (size_t)g.var * g.P;
+  double* dseeds = h->staging;
+  unsigned long long* dcounts = reinterpret_cast<unsigned long long*>(h->staging + 3 * (size_t)nseeds);
+  if ((size_t)3 * nseeds + g.var > (size_t)g.var * g.P)
+    return fail(PF_EINVAL, "too many seeds for this grid");
+  int rc = to_device(dcon, con.data(), sizeof(double) * (size_t)g.P, h->s);
+  if (rc) return rc;
+  PF_HIP(hipMemcpyAsync(dseeds, seeds.data(), sizeof(double) * seeds.size(), hipMemcpyHostToDevice, h->s));
+  hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(NB), 0, h->s, (double*)dcounts, (long)g.var);
+  h->fint_ready = false;
+  const size_t shb = sizeof(unsigned int) * (size_t)g.var;
+  const unsigned grid = grid_for(g.P, NB);
+  if (h->mode == PF_MODE_FP64_FAITHFUL) {
+    hipLaunchKernelGGL((k_voronoi_state<double>), dim3(grid), dim3(NB), shb, h->s, g, dseeds, nseeds, dcon,
+                       (double*)h->fs.main, dcounts);
+    PF_HIP(hipGetLastError());
+    PF_HIP(hipStreamSynchronize(h->s));
+    return 0;
+  }
+  if (h->mode == PF_MODE_FP64_REDUCED)
+    hipLaunchKernelGGL((k_voronoi_state<double>), dim3(grid), dim3(NB), shb, h->s, g, dseeds, nseeds, dcon,
+                       (double*)h->W_main, dcounts);
+  else
+    hipLaunchKernelGGL((k_voronoi_state<float>), dim3(grid), dim3(NB), shb, h->s, g, dseeds, nseeds, dcon,
+                       (float*)h->W_main, dcounts);
+  PF_HIP(hipGetLastError());
+  std::vector<unsigned long long> cnt(g.var);
+  PF_HIP(hipMemcpyAsync(cnt.data(), dcounts, sizeof(unsigned long long) * g.var, hipMemcpyDeviceToHost, h->s));
+  PF_HIP(hipStreamSynchronize(h->s));
+  // the means pf_put_state would take from the host arrays: c's by the same
+  // chunked compensated sum, each eta's an exact count of ones over P
+  std::vector<double> dc = host_means(nullptr, con.data(), g.P, 0);
+  dc.resize(g.F);
+  for (int v = 0; v < g.var; ++v) dc[v + 1] = (double)cnt[v] / (double)(float)g.P;
+  if (h->mode == PF_MODE_FP64_REDUCED) return finish_put_reduced<double>(h, dc);
+  return finish_put_reduced<float>(h, dc);
+}
+
 namespace {
 // Reduced modes: real state of the current step = C2R along x of a copy of W.
 template <typename R> int real_state_reduced(pf_handle* h) {
@@ -1267,6 +1474,52 @@ extern "C" int pf_set_graphs(pf_handle* h, int on) {
   return 0;
 }
 
+namespace {
+// Members that a batched launch can step: FP32-mixed, constant mobility, no
+// stage timing.
+bool batchable(const pf_handle* h) {
+  return h->mode == PF_MODE_FP32_MIXED && h->p.D_gb == 0.0 && !h->timing;
+}
+bool same_shape(const pf_handle* a, const pf_handle* b) {
+  return a->g.nx == b->g.nx && a->g.ny == b->g.ny && a->g.nz == b->g.nz && a->g.var == b->g.var;
+}
+
+// A group of batchable members of one shape: its device member table, stepped
+// on the first member's stream (the members' own streams are idle by then).
+struct Group {
+  std::vector<pf_handle*> mem;
+  Member<float>* table = nullptr;
+};
+
+int upload_table(Group& gr) {
+  std::vector<Member<float>> tab(gr.mem.size());
+  for (size_t i = 0; i < gr.mem.size(); ++i) {
+    pf_handle* h = gr.mem[i];
+    tab[i] = Member<float>{(c2<float>*)h->W_main, (c2<float>*)h->W_nyq, (c2<float>*)h->S_main,
+                           (c2<float>*)h->S_nyq, h->dc64, h->u};
+  }
+  if (hipMalloc(&gr.table, sizeof(Member<float>) * tab.size()) != hipSuccess) {
+    gr.table = nullptr;
+    return fail(PF_ENOMEM, "member table allocation failed");
+  }
+  PF_HIP(hipMemcpyAsync(gr.table, tab.data(), sizeof(Member<float>) * tab.size(), hipMemcpyHostToDevice,
+                        gr.mem[0]->s));
+  return 0;
+}
+
+// One step of every member of the group: four launches.
+int step_group(const Group& gr) {
+  pf_handle* h0 = gr.mem[0];
+  const Geo& g = h0->g;
+  const long nmem = (long)gr.mem.size();
+  int rc = xline_batch(g, h0->u, gr.table, nmem, h0->s);
+  if (!rc) rc = ypass_batch<false>(g, gr.table, nmem, h0->s);
+  if (!rc) rc = zline_batch(g, h0->u, h0->kt, gr.table, nmem, h0->s);
+  if (!rc) rc = ypass_batch<true>(g, gr.table, nmem, h0->s);
+  return rc;
+}
+}  // namespace
+
 extern "C" int pf_ensemble_step(pf_handle* const* hs, int n, int nsteps) {
   if (n < 0 || nsteps < 0 || (n > 0 && !hs)) return fail(PF_EINVAL, "bad ensemble arguments");
   for (int i = 0; i < n; ++i) {
@@ -1276,31 +1529,67 @@ extern "C" int pf_ensemble_step(pf_handle* const* hs, int n, int nsteps) {
     if (hs[i]->dev != hs[0]->dev) return fail(PF_EINVAL, "ensemble handles are on different devices");
   }
   DeviceGuard dg(n > 0 ? hs[0]->dev : 0);
-  // members step without diagnostics: a captured one-step period each
-  for (int i = 0; i < n && nsteps >= kMinGraphReplays; ++i) {
+  // groups of two or more batchable members of one shape step with one launch
+  // per stage; the rest step one by one on their own streams
+  std::vector<std::vector<pf_handle*>> groups;
+  std::vector<pf_handle*> single;
+  for (int i = 0; i < n; ++i) {
     pf_handle* h = hs[i];
-    if (h->graphs && !h->timing && h->gx_key != 0) {
-      const int rc = capture_period(h, 0);
-      if (rc) return rc;
+    if (!batchable(h)) { single.push_back(h); continue; }
+    bool placed = false;
+    for (auto& gr : groups)
+      if (same_shape(gr[0], h)) { gr.push_back(h); placed = true; break; }
+    if (!placed) groups.push_back({h});
+  }
+  std::vector<Group> batched;
+  for (auto& gm : groups) {
+    if (gm.size() < 2) single.push_back(gm[0]);
+    else batched.push_back(Group{gm, nullptr});
+  }
+  // work earlier enqueued on the members' own streams comes first
+  for (auto& gr : batched)
+    for (pf_handle* h : gr.mem) PF_HIP(hipStreamSynchronize(h->s));
+  int rc = 0;
+  for (auto& gr : batched)
+    if (!rc) rc = upload_table(gr);
+  // the groups' steps go round-robin to their streams, so groups run concurrently
+  for (int st = 0; st < nsteps && !rc; ++st)
+    for (auto& gr : batched)
+      if (!rc) rc = step_group(gr);
+  // the tables must outlive the launches that read them
+  for (auto& gr : batched) {
+    if (hipStreamSynchronize(gr.mem[0]->s) != hipSuccess && !rc) rc = fail(PF_EHIP, "batched ensemble step failed");
+    if (gr.table) (void)hipFree(gr.table);
+  }
+  if (rc) return rc;
+  for (auto& gr : batched)
+    for (pf_handle* h : gr.mem) {
+      h->fint_ready = false;
+      h->step += nsteps;
+    }
+  // members step without diagnostics: a captured one-step period each
+  for (pf_handle* h : single) {
+    if (nsteps >= kMinGraphReplays && h->graphs && !h->timing && h->gx_key != 0) {
+      const int rc1 = capture_period(h, 0);
+      if (rc1) return rc1;
     }
   }
-  for (int s = 0; s < nsteps; ++s) {
-    for (int i = 0; i < n; ++i) {
-      pf_handle* h = hs[i];
+  for (int st = 0; st < nsteps; ++st) {
+    for (pf_handle* h : single) {
       if (h->gx && h->gx_key == 0 && !h->timing && nsteps >= kMinGraphReplays) {
         PF_HIP(hipGraphLaunch(h->gx, h->s));
         h->fint_ready = false;
       } else {
-        const int rc = step_once(h, false, 0, false);
-        if (rc) return rc;
+        const int rc1 = step_once(h, false, 0, false);
+        if (rc1) return rc1;
       }
       h->step += 1;
     }
   }
   PF_HIP(hipGetLastError());
-  for (int i = 0; i < n; ++i) {
-    PF_HIP(hipStreamSynchronize(hs[i]->s));
-    tflush(hs[i]);
+  for (pf_handle* h : single) {
+    PF_HIP(hipStreamSynchronize(h->s));
+    tflush(h);
   }
   return 0;
 }

@@ -29,7 +29,8 @@ module pf_driver
 
   public :: read_input, params_of, build_gamma, write_gamma
   public :: ic_polycryst, ic_plane, ic_circle, read_restart, insert_precipitates
-  public :: write_checkpoint, write_summary_row, write_timing_row
+  public :: place_precipitates, state_extrema
+  public :: write_checkpoint, write_summary_row, write_timing_row, write_error_header, write_error_row
 
 contains
 
@@ -292,17 +293,30 @@ contains
   ! Precipitates at step == ppt_int (src/var_diff.f90:479-554): a ran_2 position
   ! is rejected when sum_{i<var} phi_i^2 > 0.8 there or when its sphere already
   ! holds phase var > 0; otherwise the sphere becomes phase var.  Rejected
-  ! positions are redrawn and do not count towards num_ppt.
-  subroutine insert_precipitates(c, phi, iseed)
+  ! positions are redrawn and do not count towards num_ppt.  centres receives
+  ! the accepted positions (3, num_ppt), in order.  The reference redraws
+  ! without bound, so it never ends when no site qualifies (sharp grain
+  ! boundaries, or no room left); here that stops with an error after
+  ! kMaxInsertDraws positions.
+  subroutine insert_precipitates(c, phi, iseed, centres)
     type(run_config), intent(in) :: c
     real(8), intent(inout) :: phi(c%var, c%nx, c%ny, c%nz)
     integer, intent(inout) :: iseed
-    integer :: i, j, k, ivar, nppt, rad_ppt, radius, ix, jy, kz, cnt
+    integer, intent(out) :: centres(3, max(c%num_ppt, 1))
+    integer, parameter :: kMaxInsertDraws = 1000000
+    integer :: i, j, k, ivar, nppt, rad_ppt, radius, ix, jy, kz, cnt, draws
     real(8) :: fsum, fi
     nppt = 1
+    draws = 0
     rad_ppt = c%ppt_rad(1)**2 + c%ppt_rad(2)**2 + c%ppt_rad(3)**2
     do while (nppt <= c%num_ppt)
       do
+        draws = draws + 1
+        if (draws > kMaxInsertDraws) then
+          write (0, '(A, I0, A, I0, A)') 'precipitate insertion: no admissible site for precipitate ', nppt, &
+            ' after ', kMaxInsertDraws, ' drawn positions'
+          error stop 1
+        end if
         ix = int(ran_2(iseed) * c%nx + 1.0)
         jy = int(ran_2(iseed) * c%ny + 1.0)
         kz = int(ran_2(iseed) * c%nz + 1.0)
@@ -327,10 +341,26 @@ contains
         if (cnt > 0) cycle
         exit
       end do
+      centres(:, nppt) = [ix, jy, kz]
+      call place_precipitates(c, phi, centres(:, nppt:nppt), 1)
+      nppt = nppt + 1
+      write (*, *) 'nppt=', nppt
+    end do
+  end subroutine
+
+  ! The spheres of radius^2 <= sum ppt_rad^2 around n centres become phase var
+  ! (the assignment of src/var_diff.f90:531-546).
+  subroutine place_precipitates(c, phi, centres, n)
+    type(run_config), intent(in) :: c
+    real(8), intent(inout) :: phi(c%var, c%nx, c%ny, c%nz)
+    integer, intent(in) :: n, centres(3, n)
+    integer :: i, j, k, ivar, m, rad_ppt, radius
+    rad_ppt = c%ppt_rad(1)**2 + c%ppt_rad(2)**2 + c%ppt_rad(3)**2
+    do m = 1, n
       do k = 1, c%nz
         do j = 1, c%ny
           do i = 1, c%nx
-            radius = (i - ix)**2 + (j - jy)**2 + (k - kz)**2
+            radius = (i - centres(1, m))**2 + (j - centres(2, m))**2 + (k - centres(3, m))**2
             if (radius <= rad_ppt) then
               phi(c%var, i, j, k) = 1.0
               do ivar = 1, c%var - 1
@@ -340,9 +370,37 @@ contains
           end do
         end do
       end do
-      nppt = nppt + 1
-      write (*, *) 'nppt=', nppt
     end do
+  end subroutine
+
+  ! The extrema the reference computes at the end of a step and prints in the
+  ! next reported row (src/var_diff.f90:1076-1094), with its initial values and
+  ! comparisons: ext = (c_max, c_min, phi_min, phi_3_max).
+  subroutine state_extrema(c, phi, con, ext)
+    type(run_config), intent(in) :: c
+    real(8), intent(in) :: phi(c%var, c%nx, c%ny, c%nz), con(c%nx, c%ny, c%nz)
+    real(8), intent(out) :: ext(4)
+    real(8) :: phi_min, phi_3, c_max, c_min, phi_sq
+    integer :: i, j, k, ivar
+    phi_min = 1.0
+    phi_3 = 0.0
+    c_max = 0.0
+    c_min = 1.0
+    do k = 1, c%nz
+      do j = 1, c%ny
+        do i = 1, c%nx
+          if (phi(c%var, i, j, k) > phi_3) phi_3 = phi(c%var, i, j, k)
+          if (con(i, j, k) <= c_min) c_min = con(i, j, k)
+          if (con(i, j, k) > c_max) c_max = con(i, j, k)
+          phi_sq = 0.0
+          do ivar = 1, c%var
+            phi_sq = phi_sq + phi(ivar, i, j, k)**2
+          end do
+          if (phi_sq < phi_min) phi_min = phi_sq
+        end do
+      end do
+    end do
+    ext = [c_max, c_min, phi_min, phi_3]
   end subroutine
 
   ! output_dir/step_NNNNNNN/{data_extract_mult.in, data_save.0001}
@@ -386,6 +444,52 @@ contains
     real(8), intent(in) :: loop_s(14), step_s, run_s
     integer :: i
     write (u, '(I8, 16(A, F8.6))') step, (' ', loop_s(i), i=1, 14), ' ', step_s, ' ', run_s
+  end subroutine
+
+  ! error_vs_fp64.dat: one row per checkpoint with, per field (c, eta_1 ..
+  ! eta_var), the largest absolute difference from the FP64 run and the root mean
+  ! square difference over the grid.
+  subroutine write_error_header(u, var)
+    integer, intent(in) :: u, var
+    integer :: i
+    write (u, '(A)', advance='no') '# step, max|c - c64|, rms(c - c64)'
+    do i = 1, var
+      write (u, '(A, I0, A, I0, A)', advance='no') ', max|eta', i, ' - eta64|, rms(eta', i, ' - eta64)'
+    end do
+    write (u, '(A)') ''
+    flush (u)
+  end subroutine
+
+  subroutine write_error_row(u, step, c, phi, con, phi64, con64)
+    integer, intent(in) :: u, step
+    type(run_config), intent(in) :: c
+    real(8), intent(in) :: phi(c%var, c%nx, c%ny, c%nz), con(c%nx, c%ny, c%nz)
+    real(8), intent(in) :: phi64(c%var, c%nx, c%ny, c%nz), con64(c%nx, c%ny, c%nz)
+    real(8) :: emax(0:c%var), ess(0:c%var), d, npts
+    integer :: i, j, k, iv
+    emax = 0.0d0
+    ess = 0.0d0
+    do k = 1, c%nz
+      do j = 1, c%ny
+        do i = 1, c%nx
+          d = abs(con(i, j, k) - con64(i, j, k))
+          emax(0) = max(emax(0), d)
+          ess(0) = ess(0) + d * d
+          do iv = 1, c%var
+            d = abs(phi(iv, i, j, k) - phi64(iv, i, j, k))
+            emax(iv) = max(emax(iv), d)
+            ess(iv) = ess(iv) + d * d
+          end do
+        end do
+      end do
+    end do
+    npts = real(c%nx, 8) * real(c%ny, 8) * real(c%nz, 8)
+    write (u, '(I8)', advance='no') step
+    do iv = 0, c%var
+      write (u, '(2(A, ES12.5))', advance='no') ' ', emax(iv), ' ', sqrt(ess(iv) / npts)
+    end do
+    write (u, '(A)') ''
+    flush (u)
   end subroutine
 
 end module pf_driver

@@ -1,8 +1,9 @@
 """GPU timestep parity against the CPU FP64 oracle (restatement of
 src/var_diff.f90:474-1118) on seeded inputs small enough for the oracle.
 
-Tolerances: FP64 forms 1e-11 relative per field (max-norm) and 1e-10 relative
-on the summary energies after the stated number of steps; FP32-mixed 1e-4.
+Tolerances (about 30x / 4x the largest difference measured over these shapes:
+FP64 3.7e-15, FP32 4.7e-7 per field): FP64 forms 1e-13 relative per field
+(max-norm), on the summary energies and on the extrema; FP32-mixed 2e-6.
 """
 import numpy as np
 import pytest
@@ -16,6 +17,16 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def energy_scale(o, k):
+    """The magnitude a summary energy's error is relative to: its own for the
+    three sums, and for e_tot = f_int + f_ch + f_eta the sum of their magnitudes
+    (the terms can nearly cancel: e_tot = -16 from terms of ~1e4 at 512 x 16 x 16
+    x 37, where each term's rounding is what e_tot inherits)."""
+    if k == "e_tot":
+        return max(abs(o["f_int"]) + abs(o["f_ch"]) + abs(o["f_eta"]), 1.0)
+    return max(abs(o[k]), 1.0)
+
+
 def _compare(pf, oracle, shape, var, nseeds, nsteps, mode, params, tol_state, tol_e):
     nx, ny, nz = shape
     phi, con = pf.ic_voronoi(nx, ny, nz, var, nseeds)
@@ -25,7 +36,12 @@ def _compare(pf, oracle, shape, var, nseeds, nsteps, mode, params, tol_state, to
         gphi, gcon = sim.get_state()
     ophi, ocon = phi.copy(), con.copy()
     od = oracle.run(oracle.make_params(**params), ophi, ocon, nsteps)
-    print(f"{mode} {shape} var={var} steps={nsteps}: phi {_rel(gphi, ophi):.2e} con {_rel(gcon, ocon):.2e}")
+    e_worst = max((abs(d[k] - od[d["step"] - 1][k]) / energy_scale(od[d["step"] - 1], k)
+                   for d in dg for k in ("f_int", "f_ch", "f_eta", "e_tot")), default=0.0)
+    x_worst = max((abs(d[k] - od[d["step"] - 1][k]) for d in dg
+                   for k in ("c_max", "c_min", "phi_min", "phi_3_max")), default=0.0)
+    print(f"{mode} {shape} var={var} steps={nsteps}: phi {_rel(gphi, ophi):.2e} con {_rel(gcon, ocon):.2e} "
+          f"energies {e_worst:.2e} extrema {x_worst:.2e}")
     assert np.isfinite(gphi).all() and np.isfinite(gcon).all()
     assert _rel(gphi, ophi) < tol_state
     assert _rel(gcon, ocon) < tol_state
@@ -33,36 +49,36 @@ def _compare(pf, oracle, shape, var, nseeds, nsteps, mode, params, tol_state, to
     for d in dg:
         o = od[d["step"] - 1]
         for k in ("f_int", "f_ch", "f_eta", "e_tot"):
-            assert abs(d[k] - o[k]) <= tol_e * max(abs(o[k]), 1.0), (k, d[k], o[k])
+            assert abs(d[k] - o[k]) <= tol_e * energy_scale(o, k), (k, d[k], o[k])
         for k in ("c_max", "c_min", "phi_min", "phi_3_max"):
-            assert abs(d[k] - o[k]) <= tol_e * 10 + tol_state, (k, d[k], o[k])
+            assert abs(d[k] - o[k]) <= tol_state, (k, d[k], o[k])
 
 
 @pytest.mark.parametrize("shape,var,nseeds", [((16, 16, 16), 3, 6), ((24, 16, 32), 4, 8),
                                               ((32, 32, 32), 16, 16)])
 @pytest.mark.parametrize("dgb", [0.0, 0.3])
 def test_fp64_faithful_vs_oracle(pf, oracle, shape, var, nseeds, dgb):
-    _compare(pf, oracle, shape, var, nseeds, 10, "fp64_faithful", dict(VOR, D_gb=dgb), 1e-11, 1e-10)
+    _compare(pf, oracle, shape, var, nseeds, 10, "fp64_faithful", dict(VOR, D_gb=dgb), 1e-13, 1e-13)
 
 
 @pytest.mark.parametrize("shape,var,nseeds", [((16, 16, 16), 3, 6), ((24, 16, 32), 4, 8),
                                               ((32, 32, 32), 16, 16)])
 @pytest.mark.parametrize("dgb", [0.0, 0.3])
 def test_fp64_reduced_vs_oracle(pf, oracle, shape, var, nseeds, dgb):
-    _compare(pf, oracle, shape, var, nseeds, 10, "fp64_reduced", dict(VOR, D_gb=dgb), 1e-11, 1e-10)
+    _compare(pf, oracle, shape, var, nseeds, 10, "fp64_reduced", dict(VOR, D_gb=dgb), 1e-13, 1e-13)
 
 
 @pytest.mark.parametrize("dgb", [0.0, 0.3])
 def test_fp32_mixed_vs_oracle(pf, oracle, dgb):
-    _compare(pf, oracle, (32, 32, 32), 4, 8, 20, "fp32_mixed", dict(VOR, D_gb=dgb), 1e-4, 1e-4)
+    _compare(pf, oracle, (32, 32, 32), 4, 8, 20, "fp32_mixed", dict(VOR, D_gb=dgb), 2e-6, 2e-6)
 
 
 # One or two order parameters: the h sum runs over no field (var = 1) or one
 # (var = 2), the gamma matrix has no sigma_1 pair, and the x-line blocks hold 2
 # or 3 fields.
 @pytest.mark.parametrize("var", [1, 2])
-@pytest.mark.parametrize("mode,tol_s,tol_e", [("fp64_faithful", 1e-11, 1e-10), ("fp64_reduced", 1e-11, 1e-10),
-                                              ("fp32_mixed", 1e-4, 1e-4)])
+@pytest.mark.parametrize("mode,tol_s,tol_e", [("fp64_faithful", 1e-13, 1e-13), ("fp64_reduced", 1e-13, 1e-13),
+                                              ("fp32_mixed", 2e-6, 2e-6)])
 def test_few_order_parameters_vs_oracle(pf, oracle, var, mode, tol_s, tol_e):
     _compare(pf, oracle, (16, 24, 16), var, 4, 10, mode, dict(VOR, D_gb=0.3), tol_s, tol_e)
 
@@ -113,7 +129,7 @@ def test_fp32_long_x_lines_many_fields(pf, oracle):
     """x lines of 512 points with 37 fields: one 1024-thread block per line and
     80 KiB of LDS (the 512^3 x 37 configuration's x-line shape, on a 512 x 16 x 16
     grid the oracle can follow)."""
-    _compare(pf, oracle, (512, 16, 16), 36, 40, 6, "fp32_mixed", dict(VOR, D_gb=0.0), 1e-4, 1e-4)
+    _compare(pf, oracle, (512, 16, 16), 36, 40, 6, "fp32_mixed", dict(VOR, D_gb=0.0), 2e-6, 2e-6)
 
 
 def test_fp64_reduced_too_many_fields_is_an_error(pf):
@@ -145,24 +161,36 @@ def test_fp64_500_steps_vs_oracle(pf, oracle, mode):
 
 
 def test_fp32_error_after_1000_steps(pf):
-    """FP32-mixed against FP64 reduced over 1000 steps of a 32^3 x 5-field
-    polycrystal.  Bound 3e-5 per field (measured: 1.5e-5 in c, <= 1e-5 in eta;
-    the same comparison on the 256^3 x 17 benchmark workload gives 5.5e-6).  The
-    FP32 state multiplier is applied as a float hi+lo pair, so its rounding does
-    not bias every step the same way; without that the error grows to ~1e-4."""
+    """FP32-mixed against FP64 over 1000 steps of a 32^3 x 5-field polycrystal,
+    printed every 100 steps next to the difference between the two FP64 forms
+    (which differ only in rounding, ~1e-16 per step): the ratio of the two
+    curves is the rounding unit's ratio, so both measure how strongly these
+    dynamics amplify rounding.  Bound 3e-5 per field (measured 1.3e-5 in c;
+    the benchmark workload at 256^3 x 17 stays below 1e-5, test_gpu_configs).
+    The FP32 state multiplier is applied as a float hi+lo pair, so its rounding
+    does not bias every step the same way; without that the error grows to ~1e-4."""
     n, var = 32, 4
     phi, con = pf.ic_voronoi(n, n, n, var, 8)
     prm = pf.make_params(**VOR)
-    out = {}
-    for mode in ("fp32_mixed", "fp64_reduced"):
-        with pf.Simulation(prm, n, n, n, var, mode) as sim:
-            sim.put_state(phi, con)
-            sim.step(1000)
-            out[mode] = sim.get_state()
-    (pa, ca), (pb, cb) = out["fp32_mixed"], out["fp64_reduced"]
-    errs = [float(np.abs(ca - cb).max())] + [float(np.abs(pa[..., i] - pb[..., i]).max()) for i in range(var)]
-    print("fp32 vs fp64 after 1000 steps, per-field max", ["%.1e" % e for e in errs])
-    assert max(errs) <= 3e-5
+    sims = {m: pf.Simulation(prm, n, n, n, var, m) for m in ("fp32_mixed", "fp64_reduced", "fp64_faithful")}
+    try:
+        for s in sims.values():
+            s.put_state(phi, con)
+        for ck in range(1, 11):
+            st = {}
+            for m, s in sims.items():
+                s.step(100)
+                st[m] = s.get_state()
+            (pa, ca), (pb, cb), (pc, cc) = st["fp32_mixed"], st["fp64_reduced"], st["fp64_faithful"]
+            e32 = [float(np.abs(ca - cb).max())] + [float(np.abs(pa[..., i] - pb[..., i]).max()) for i in range(var)]
+            e64 = [float(np.abs(cc - cb).max())] + [float(np.abs(pc[..., i] - pb[..., i]).max()) for i in range(var)]
+            print(f"32^3 step {ck * 100}: |fp32 - fp64| {max(e32):.2e} (c {e32[0]:.2e}), "
+                  f"|fp64 faithful - reduced| {max(e64):.2e}, ratio {max(e32) / max(max(e64), 1e-300):.2e}")
+    finally:
+        for s in sims.values():
+            s.close()
+    print("fp32 vs fp64 after 1000 steps, per-field max", ["%.1e" % e for e in e32])
+    assert max(e32) <= 3e-5
 
 
 def test_ensemble_step_equals_individual_steps(pf):
@@ -195,6 +223,44 @@ def test_ensemble_step_equals_individual_steps(pf):
             pf.ensemble_step([s, s], 1)  # a handle may appear once
 
 
+def test_batched_ensemble_equals_individual_steps(pf):
+    """Members of one grid shape in the FP32-mixed form with constant mobility
+    step together, one launch per stage over all of them, each with its own
+    parameters (kappa, M, L as a parameter sweep draws them); every member's
+    result is bit-identical to stepping it alone.  Two shapes form two groups;
+    a member with D_gb != 0 and a faithful member step on their own."""
+    cases = [(16, 16, 16, 3, 0.5, 0.8, 1.3, 0.0), (16, 16, 16, 3, 1.7, 1.2, 0.6, 0.0),
+             (16, 16, 16, 3, 0.9, 0.5, 1.9, 0.0), (24, 16, 32, 4, 1.1, 1.4, 0.7, 0.0),
+             (24, 16, 32, 4, 0.3, 1.9, 1.2, 0.0), (16, 16, 16, 3, 1.0, 1.0, 1.0, 0.2)]
+    prms = [pf.make_params(t_step=0.005, grad_coeff_phi=k, grad_coeff_c=k, D_L=m, D_S=m, mob_phi=l, D_gb=d)
+            for (_, _, _, _, k, m, l, d) in cases]
+    ics_ = [pf.ic_voronoi(nx, ny, nz, var, 5, -(i + 1)) for i, (nx, ny, nz, var, *_r) in enumerate(cases)]
+    alone = []
+    for (nx, ny, nz, var, *_r), prm, (phi, con) in zip(cases, prms, ics_):
+        with pf.Simulation(prm, nx, ny, nz, var, "fp32_mixed") as s:
+            s.put_state(phi, con)
+            s.step(9)
+            alone.append((s.get_state(), s.diagnose()))
+    sims = []
+    try:
+        for (nx, ny, nz, var, *_r), prm, (phi, con) in zip(cases, prms, ics_):
+            s = pf.Simulation(prm, nx, ny, nz, var, "fp32_mixed")
+            s.put_state(phi, con)
+            sims.append(s)
+        with pf.Simulation(pf.make_params(**VOR), 16, 16, 16, 3, "fp64_faithful") as f64:
+            f64.put_state(*ics_[0])
+            pf.ensemble_step(sims + [f64], 4)
+            pf.ensemble_step(sims + [f64], 5)
+        for s, ((pa, ca), da) in zip(sims, alone):
+            pb, cb = s.get_state()
+            assert np.array_equal(pa, pb) and np.array_equal(ca, cb)
+            db = s.diagnose()
+            assert db["step"] == da["step"] == 10 and db["e_tot"] == da["e_tot"]
+    finally:
+        for s in sims:
+            s.close()
+
+
 # Every supported axis length through the fused kernels (x lines, y panels, z
 # columns), small enough in the other axes for the oracle.
 AXIS_SHAPES = [(8, 12, 16), (12, 8, 24), (48, 16, 8), (96, 8, 12), (160, 8, 8), (192, 12, 8),
@@ -202,36 +268,11 @@ AXIS_SHAPES = [(8, 12, 16), (12, 8, 24), (48, 16, 8), (96, 8, 12), (160, 8, 8), 
 
 
 @pytest.mark.parametrize("shape", AXIS_SHAPES, ids=lambda s: "x".join(map(str, s)))
-@pytest.mark.parametrize("mode,tol_s,tol_e", [("fp64_reduced", 1e-11, 1e-10), ("fp32_mixed", 1e-4, 1e-4)])
+@pytest.mark.parametrize("mode,tol_s,tol_e", [("fp64_reduced", 1e-13, 1e-13), ("fp32_mixed", 2e-6, 2e-6)])
 def test_axis_lengths_vs_oracle(pf, oracle, shape, mode, tol_s, tol_e):
     _compare(pf, oracle, shape, 4, 6, 4, mode, dict(VOR, D_gb=0.25), tol_s, tol_e)
 
 
 @pytest.mark.parametrize("shape", [(160, 8, 8), (12, 96, 8), (8, 12, 192)], ids=lambda s: "x".join(map(str, s)))
 def test_axis_lengths_faithful_vs_oracle(pf, oracle, shape):
-    _compare(pf, oracle, shape, 3, 5, 4, "fp64_faithful", dict(VOR, D_gb=0.25), 1e-11, 1e-10)
-
-
-# The benchmark workload at its full size (256^3, c + 16 eta), where the oracle
-# is too slow: size-independent properties of the step instead.  The Voronoi
-# start has sharp interfaces, so the total energy falls steeply over these 40
-# steps (measured -5.1e5 -> -1.6e6; the oracle falls the same way at 32^3), far
-# above FP32 rounding.
-@pytest.mark.parametrize("mode,tol_mass", [("fp32_mixed", 1e-6), ("fp64_reduced", 1e-12)])
-def test_full_size_mass_conserved_energy_decreasing(pf, mode, tol_mass):
-    n, var = 256, 16
-    phi, con = pf.ic_voronoi(n, n, n, var, 64)
-    m0 = float(con.mean())
-    with pf.Simulation(pf.make_params(**VOR), n, n, n, var, mode) as sim:
-        sim.put_state(phi, con)
-        rows = sim.step(40, diag_every=2)
-        gphi, gcon = sim.get_state()
-    print(f"{mode} 256^3 x 17: mass drift {abs(gcon.mean() - m0) / m0:.2e}, "
-          f"e_tot {rows[0]['e_tot']:.6e} -> {rows[-1]['e_tot']:.6e}")
-    assert np.isfinite(gphi).all() and np.isfinite(gcon).all()
-    assert abs(float(gcon.mean()) - m0) <= tol_mass * m0
-    assert len(rows) == 20
-    for r in rows:
-        assert abs(r["c_mean"] - m0) <= tol_mass * m0, (r["step"], r["c_mean"], m0)
-    e = [r["e_tot"] for r in rows]
-    assert all(b < a for a, b in zip(e, e[1:])), e
+    _compare(pf, oracle, shape, 3, 5, 4, "fp64_faithful", dict(VOR, D_gb=0.25), 1e-13, 1e-13)
