@@ -121,6 +121,16 @@ int pf_fft_r2c(int nx, int ny, int nz, int batch, int prec, const void* d_in, vo
                int layout, double scale, void* stream);
 int pf_fft_c2r(int nx, int ny, int nz, int batch, int prec, const void* d_in, void* d_out,
                int layout, void* stream);
+/* pf_fft_r2c without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  Its scratch is a static device array of 64 MiB per
+ * device (the split-layout copy of the batch: 8*P*batch + 16*ny*nz*batch bytes
+ * in FP64, half that in FP32); larger requests return PF_EINVAL.  Every call on
+ * a device shares that device's array, so calls that overlap on one device race.
+ * The first call on a device sets up (device addresses of the static scratch and
+ * twiddle tables, and one copy of the tables); make that call before capturing. */
+int pf_fft_r2c_noalloc(int nx, int ny, int nz, int batch, int prec, const void* d_in, void* d_out,
+                       int layout, double scale, void* stream);
 /* Host-buffer FP64 drop-ins for f_trans(in,out,Nx,Ny,Nz,...) (src/f_trans.f90:1)
  * and inv_trans(in,out,...) (src/inv_trans.f90:1); spectrum in PF_LAYOUT_ZFAST. */
 int pf_f_trans(const double* in, double* out, int nx, int ny, int nz);

@@ -67,7 +67,7 @@ EXPORTED = (
     "pf_set_timing", "pf_stage_times", "pf_stage_name", "pf_stage_bytes", "pf_stream",
     "pf_fft_r2c", "pf_fft_c2r", "pf_f_trans", "pf_inv_trans", "pf_ran2_init", "pf_ran2",
     "pf_k_space", "pf_gamma", "pf_ic_random", "pf_ic_voronoi", "pf_last_error", "pf_version",
-    "pf_ensemble_step", "pf_analyze_state", "pf_set_graphs", "pf_put_voronoi",
+    "pf_ensemble_step", "pf_analyze_state", "pf_set_graphs", "pf_put_voronoi", "pf_fft_r2c_noalloc",
 )
 
 

@@ -587,6 +587,58 @@ extern "C" int pf_fft_c2r(int nx, int ny, int nz, int batch, int prec, const voi
   return fail(PF_EINVAL, "bad precision");
 }
 
+// ---------------------------------------------------------------------------
+// pf_fft_r2c_noalloc: pf_fft_r2c with its scratch in static device memory.
+// scratch is this file's array; in host code its name only stands for the
+// symbol, so every use goes through the address hipGetSymbolAddress returns for
+// the current device (each device holds its own copy).
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch;  // device -> the address of its copy of scratch
+
+// Set-up, once per device with that device current: the device address of
+// scratch (and, through twiddle_table, the static twiddle tables).
+void* scratch_ptr() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(dev);
+  if (it != g_scratch.end()) return it->second;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  g_scratch[dev] = ptr;
+  return ptr;
+}
+}  // namespace
+
+extern "C" int pf_fft_r2c_noalloc(int nx, int ny, int nz, int batch, int prec, const void* d_in, void* d_out,
+                                  int layout, double scale, void* stream) {
+  int rc = check_dims(nx, ny, nz, batch);
+  if (rc) return rc;
+  if (!d_in || !d_out) return fail(PF_EINVAL, "null buffer");
+  if (layout != PF_LAYOUT_XFAST && layout != PF_LAYOUT_ZFAST) return fail(PF_EINVAL, "bad layout");
+  if (prec != PF_PREC_FP64 && prec != PF_PREC_FP32) return fail(PF_EINVAL, "bad precision");
+  const size_t need = prec == PF_PREC_FP64 ? r2c_scratch_bytes<double>(nx, ny, nz, batch)
+                                           : r2c_scratch_bytes<float>(nx, ny, nz, batch);
+  if (need > kNoAllocScratchBytes)
+    return fail(PF_EINVAL, "pf_fft_r2c_noalloc: " + std::to_string(batch) + " field(s) of " + std::to_string(nx) +
+                               "x" + std::to_string(ny) + "x" + std::to_string(nz) + " need " +
+                               std::to_string(need) + " bytes of scratch; the limit is " +
+                               std::to_string(kNoAllocScratchBytes));
+  void* ws = scratch_ptr();
+  if (!ws) return fail(PF_EHIP, "pf_fft_r2c_noalloc: no device address for the static scratch");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == PF_PREC_FP64) return r2c_core<double>(nx, ny, nz, batch, d_in, d_out, layout, scale, ws, s);
+  return r2c_core<float>(nx, ny, nz, batch, d_in, d_out, layout, scale, ws, s);
+}
+
 // Device buffers of pf_f_trans / pf_inv_trans, per device, grown as needed and
 // kept until the process ends.  These calls are synchronous: the lock is held
 // for the whole call, so calls from several threads take turns.
