@@ -41,27 +41,53 @@ def workload_params(pf):
                           con_0_mat=0.1, con_0_ppt=1.0)
 
 
-def run_mode(pf, torch, mode, n, var, phi, con, steps, warmup, timing=True, graphs=True, stages_after=False):
-    """Times `steps` steps (summary rows every 2nd step) after `warmup`.  With
-    timing, per-stage HIP events are recorded inside the timed region (and the
-    step is launched kernel by kernel); with stages_after, the timed region runs
-    without events (graph replay when graphs) and the stage times come from a
-    second run of the same length afterwards."""
+def run_mode(pf, torch, mode, n, var, seeds, steps, warmup, graphs=True):
+    """Times `steps` steps (summary rows every 2nd step) after `warmup`, without
+    timing events (graph replay when graphs), from the device-built start."""
     with pf.Simulation(workload_params(pf), n, n, n, var, mode) as sim:
         sim.set_graphs(graphs)
-        sim.put_state(phi, con)
+        sim.put_voronoi(seeds, -1, 0.4, 0.02)
         sim.step(warmup, diag_every=2)
-        sim.set_timing(timing and not stages_after)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         sim.step(steps, diag_every=2)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        if timing and stages_after:
-            sim.set_timing(True)
-            sim.step(steps, diag_every=2)
+    return dt
+
+
+def run_headline(pf, torch, n, var, seeds, steps, warmup, graphs=True):
+    """The headline FP32 run.  The warm-up times every stage (HIP events on the
+    library's stream) and so names the dominant kernel; the timed region then
+    records events around that kernel's launches only (two per step, instead of
+    two per kernel), which give its average launch time measured live; a second
+    run of the same length afterwards gives the per-stage breakdown, and a third,
+    with no events at all, the graph-replayed step time."""
+    with pf.Simulation(workload_params(pf), n, n, n, var, "fp32_mixed") as sim:
+        sim.set_graphs(graphs)
+        # the polycrystal built on the device: the state pf.ic_voronoi + put_state gives
+        sim.put_voronoi(seeds, -1, 0.4, 0.02)
+        sim.set_timing(True)
+        sim.step(warmup, diag_every=2)
+        dom = max(sim.stage_times().items(), key=lambda kv: kv[1]["ms"])[0]
+        sim.set_timing_stages([dom])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.step(steps, diag_every=2)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        dom_stats = sim.stage_times()[dom]
+        sim.set_timing(True)
+        sim.step(steps, diag_every=2)
         stages = sim.stage_times()
-    return dt, stages
+        sim.set_timing(False)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sim.step(steps, diag_every=2)
+        torch.cuda.synchronize()
+        dt_graph = time.perf_counter() - t1
+        tuning = sim.get_tuning()
+    return dt, dom, dom_stats, stages, dt_graph, tuning
 
 
 def stream_reference(torch, nbytes):
@@ -91,31 +117,49 @@ def roofline_of(stages, steps):
     return name, avg_ms, achieved, st["bytes"]
 
 
-def stage_kernels(n):
-    """Kernel behind each timed stage of the FP32 step on an n^3 grid (names as
-    rocprofv3 prints them; the x-line's lines-per-block argument is shown only
-    at 256^3, where it is 1)."""
+def stage_kernels(n, lines=1):
+    """Kernel behind each timed stage of the FP32 step on an n^3 grid, named as
+    rocprofv3 prints them (lines: x lines per x-line block)."""
     return {
-        "nonlinear": [f"pf::k_xline<float, {n}, 1, false>" if n == 256 else f"pf::k_xline<float, {n}, ...>"],
-        "y_fwd": [f"pf::k_ypass<float, {n}, false>"],
-        "kspace_update": [f"pf::k_zline<float, {n}, 0, true, false>", f"pf::k_zline<float, {n}, 0, true, true>"],
-        "y_inv": [f"pf::k_ypass<float, {n}, true>"],
+        "nonlinear": [f"pf::k_xline<float, {n}, {lines}, false, false>"],
+        "y_fwd": [f"pf::k_ypass<float, {n}, false, false>"],
+        "kspace_update": [f"pf::k_zline<float, {n}, 0, true, false, false>",
+                          f"pf::k_zline<float, {n}, 0, true, true, false>"],
+        "y_inv": [f"pf::k_ypass<float, {n}, true, false>"],
     }
 
 
 STAGE_KERNELS = stage_kernels(256)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fused_kernels_256x17_fp32.md")
+# The sources that decide what the step kernels read and write and how they are
+# launched; the traffic profile records their hash when it is measured.
+KERNEL_SOURCES = [os.path.join(ROOT, "phase-field-microstructure-evolution_amd", "csrc", f)
+                  for f in ("pf_fused.hpp", "pf_kernels.hpp", "pf_fft2.hpp", "pf_fft.hpp", "pf_step_kernels.hpp",
+                            "pf_step.hip", "pf_tuned.inc")]
+
+
+def kernel_source_sha16():
+    import hashlib
+
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def measured_traffic(stage, n, var):
     """HBM bytes per launch of the stage's kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), or None when that
-    summary does not cover this workload."""
+    summary does not cover this workload or was measured on other kernel
+    sources than these (its kernel_source_sha16 differs)."""
     if n != 256 or var != 16 or stage not in STAGE_KERNELS or not os.path.exists(TRAFFIC_FILE):
         return None
     txt = open(TRAFFIC_FILE).read()
     try:
         blob = json.loads(txt.split("```json", 1)[1].split("```", 1)[0])
+        if blob.get("kernel_source_sha16") != kernel_source_sha16():
+            return None
         vals = [blob["kernels"][k]["traffic_bytes"] for k in STAGE_KERNELS[stage]]
     except (IndexError, KeyError, ValueError):
         return None
@@ -295,9 +339,6 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel instead of replaying captured step periods")
-    ap.add_argument("--stages-after", action="store_true",
-                    help="time the steps without per-stage events (graph replay) and take the "
-                         "stage times from a second run afterwards")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes) and exit")
@@ -329,8 +370,11 @@ def main():
         ensemble(pf, torch, args)
         return
     n, var = args.size, args.var
-    log(f"building Voronoi IC {n}^3 x {var + 1} fields, {args.seeds} seeds")
-    phi, con = pf.ic_voronoi(n, n, n, var, args.seeds, -1, 0.4, 0.02)
+    phi = con = None
+    if args.accuracy or args.boundary or not args.no_cpu:
+        # host arrays of the same start, for the legs that need them
+        log(f"building Voronoi IC {n}^3 x {var + 1} fields, {args.seeds} seeds on the host")
+        phi, con = pf.ic_voronoi(n, n, n, var, args.seeds, -1, 0.4, 0.02)
 
     if args.accuracy:
         accuracy(pf, n, var, phi, con, args)
@@ -340,13 +384,17 @@ def main():
         return
 
     log(f"fp32_mixed: {args.warmup} warmup + {args.steps} timed steps")
-    dt, stages = run_mode(pf, torch, "fp32_mixed", n, var, phi, con, args.steps, args.warmup,
-                          graphs=not args.no_graphs, stages_after=args.stages_after)
+    dt, name, dom, stages, dt_graph, tuning = run_headline(pf, torch, n, var, args.seeds, args.steps, args.warmup,
+                                                           graphs=not args.no_graphs)
     ms_step = dt / args.steps * 1e3
     gpups = n ** 3 * args.steps / dt
-    name, avg_ms, achieved, kbytes = roofline_of(stages, args.steps)
+    avg_ms = dom["ms"] / max(dom["count"], 1)
+    kbytes = dom["bytes"]
+    achieved = kbytes / (avg_ms * 1e-3) / 1e9 if kbytes > 0 else 0.0
+    lines = tuning["xline_lines"] or 1
     log(f"fp32_mixed: {ms_step:.3f} ms/step, {gpups:.3e} GPUPS; dominant stage {name} "
-        f"{avg_ms:.3f} ms avg, {achieved:.0f} GB/s")
+        f"{avg_ms:.3f} ms avg ({dom['count']} launches timed), {achieved:.0f} GB/s; "
+        f"graph replay without events {dt_graph / args.steps * 1e3:.3f} ms/step")
     stage_summary = {k: dict(ms_per_step=v["ms"] / max(args.steps, 1),
                              GBs=(v["bytes"] / (v["ms"] / v["count"] * 1e-3) / 1e9)
                              if v["count"] and v["bytes"] else None)
@@ -361,8 +409,7 @@ def main():
     if not args.no_fp64:
         for mode in ("fp64_reduced", "fp64_faithful"):
             log(f"{mode}: 2 warmup + {args.fp64_steps} timed steps")
-            d64, _ = run_mode(pf, torch, mode, n, var, phi, con, args.fp64_steps, 2, timing=False,
-                              graphs=not args.no_graphs)
+            d64 = run_mode(pf, torch, mode, n, var, args.seeds, args.fp64_steps, 2, graphs=not args.no_graphs)
             fp64[mode] = dict(value=n ** 3 * args.fp64_steps / d64, ms_per_step=d64 / args.fp64_steps * 1e3)
             log(f"{mode}: {fp64[mode]['ms_per_step']:.3f} ms/step")
 
@@ -378,6 +425,7 @@ def main():
     Br = n ** 3 * 4.0
     Bc = 2.0 * (n // 2 + 1) * n * n * 4.0
     model_bytes = F * (4 * Br + 13 * Bc)
+    fused_bytes = sum(v["bytes"] * v["count"] for v in stages.values() if v["bytes"]) / max(args.steps, 1)
     out = {
         "metric": METRIC,
         "value": gpups,
@@ -399,15 +447,24 @@ def main():
         "field_updates_per_s": gpups * F,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name, n, var),
-                     "stage": name, "kernel": " / ".join(stage_kernels(n).get(name, [name])),
+                     "kernel_source_sha16": kernel_source_sha16(),
+                     "stage": name, "kernel": " / ".join(stage_kernels(n, lines).get(name, [name])),
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": kbytes,
                      "stream_reference": {"achieved": stream, "frac": achieved / stream if stream else None,
                                           "kernel": "torch in-place multiply over the same bytes "
                                                     "(read + write), same run"}},
-        "step_model": {"model_bytes_per_step": model_bytes,
-                       "model_GBs": model_bytes / (ms_step * 1e-3) / 1e9,
-                       "model_frac": model_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        # fused: the algorithmic bytes our four kernels move per step over the
+        # step time (the whole step's HBM fraction); reference sequence: the
+        # bytes the reference's unfused pass sequence would move, over the same
+        # time -- a speed-equivalent, not a bandwidth any kernel achieved
+        "step_model": {"fused_bytes_per_step": fused_bytes,
+                       "fused_GBs": fused_bytes / (ms_step * 1e-3) / 1e9,
+                       "fused_frac": fused_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "reference_sequence_bytes_per_step": model_bytes,
+                       "reference_sequence_equivalent_GBs": model_bytes / (ms_step * 1e-3) / 1e9},
         "stages": stage_summary,
+        "ms_per_step_graph_replay": dt_graph / args.steps * 1e3,
+        "launch_shapes": tuning,
         "fp64": fp64,
         "cpu_baseline": cpu,
     }

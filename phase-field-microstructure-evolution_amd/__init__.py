@@ -59,6 +59,14 @@ class Ran2State(C.Structure):
     _fields_ = [("idum2", C.c_int), ("iy", C.c_int), ("iv", C.c_int * 32)]
 
 
+TUNING_NAMES = ("xline_lines", "xline_grid", "zline_grid", "zline_run_rows", "zline_plan")
+TUNING_DEFAULTS = dict(xline_lines=0, xline_grid=0, zline_grid=0, zline_run_rows=-1, zline_plan=0)
+
+
+class Tuning(C.Structure):
+    _fields_ = [(n, C.c_int) for n in TUNING_NAMES]
+
+
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
@@ -68,6 +76,7 @@ EXPORTED = (
     "pf_fft_r2c", "pf_fft_c2r", "pf_f_trans", "pf_inv_trans", "pf_ran2_init", "pf_ran2",
     "pf_k_space", "pf_gamma", "pf_ic_random", "pf_ic_voronoi", "pf_last_error", "pf_version",
     "pf_ensemble_step", "pf_analyze_state", "pf_set_graphs", "pf_put_voronoi", "pf_fft_r2c_noalloc",
+    "pf_set_tuning", "pf_get_tuning", "pf_set_timing_stages",
 )
 
 
@@ -103,6 +112,9 @@ def lib():
     L.pf_diagnose.argtypes = [vp, C.POINTER(Diag)]
     L.pf_set_timing.argtypes = [vp, ip]
     L.pf_set_graphs.argtypes = [vp, ip]
+    L.pf_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.pf_set_timing_stages.argtypes = [vp, C.c_uint]
+    L.pf_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.pf_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_long), ip]
     L.pf_stage_name.argtypes = [ip, C.c_char_p, ip]
     L.pf_stage_bytes.argtypes = [vp, ip]
@@ -219,9 +231,35 @@ class Simulation:
     def set_timing(self, on: bool) -> None:
         _check(lib().pf_set_timing(self._h, int(on)))
 
+    def set_timing_stages(self, names) -> None:
+        """Timing events for the named stages only (names as stage_times reports them)."""
+        mask = 0
+        for i in range(32):
+            buf = C.create_string_buffer(64)
+            if lib().pf_stage_name(i, buf, 64) < 0:
+                break
+            if buf.value.decode() in names:
+                mask |= 1 << i
+        _check(lib().pf_set_timing_stages(self._h, mask))
+
     def set_graphs(self, on: bool) -> None:
         """Replay captured step periods (hipGraph) or launch every kernel."""
         _check(lib().pf_set_graphs(self._h, int(on)))
+
+    def set_tuning(self, **kw) -> None:
+        """Launch shapes of the fused kernels (pf_set_tuning): xline_lines,
+        xline_grid, zline_grid, zline_run_rows, zline_plan; unnamed ones take
+        their defaults."""
+        unknown = set(kw) - set(TUNING_NAMES)
+        if unknown:
+            raise TypeError(f"unknown tuning fields {sorted(unknown)}")
+        t = Tuning(**dict(TUNING_DEFAULTS, **kw))
+        _check(lib().pf_set_tuning(self._h, C.byref(t)))
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().pf_get_tuning(self._h, C.byref(t)))
+        return {n: getattr(t, n) for n in TUNING_NAMES}
 
     def stage_times(self):
         n = 32

@@ -71,7 +71,37 @@ __host__ __device__ constexpr int plan_fixed(int N, int p) {
     default: return 0;
   }
 }
-__host__ __device__ constexpr int plan_rad(int N, int p) {
+// Alternative radix sequences (plan variants V = 1, 2) for the lengths of the
+// z-line transform, candidates of the tuner: V = 1 uses fewer threads per line
+// (larger first radix), V = 2 more threads with fewer registers each.  0 = no
+// alternative for that length (the variant is then the default plan).
+__host__ __device__ constexpr int plan_alt(int N, int V, int p) {
+  if (V == 1) {
+    switch (N) {
+      case 64: { const int r[2] = {16, 4}; return p < 2 ? r[p] : 1; }
+      case 128: { const int r[2] = {8, 16}; return p < 2 ? r[p] : 1; }
+      case 160: { const int r[2] = {16, 10}; return p < 2 ? r[p] : 1; }
+      case 192: { const int r[2] = {16, 12}; return p < 2 ? r[p] : 1; }
+      case 256: { const int r[3] = {16, 4, 4}; return p < 3 ? r[p] : 1; }
+      case 512: { const int r[3] = {16, 8, 4}; return p < 3 ? r[p] : 1; }
+      default: return 0;
+    }
+  }
+  if (V == 2) {
+    switch (N) {
+      case 64: { const int r[2] = {4, 16}; return p < 2 ? r[p] : 1; }
+      case 128: { const int r[3] = {8, 4, 4}; return p < 3 ? r[p] : 1; }
+      case 160: { const int r[2] = {10, 16}; return p < 2 ? r[p] : 1; }
+      case 192: { const int r[2] = {12, 16}; return p < 2 ? r[p] : 1; }
+      case 256: { const int r[3] = {8, 8, 4}; return p < 3 ? r[p] : 1; }
+      case 512: { const int r[3] = {16, 16, 2}; return p < 3 ? r[p] : 1; }
+      default: return 0;
+    }
+  }
+  return 0;
+}
+__host__ __device__ constexpr int plan_rad(int N, int p, int V = 0) {
+  if (V > 0 && plan_alt(N, V, 0)) return plan_alt(N, V, p);
   if (plan_fixed(N, 0)) return plan_fixed(N, p);
   const int order[9] = {16, 12, 10, 8, 6, 5, 4, 3, 2};
   int n = N, idx = 0;
@@ -94,32 +124,32 @@ __host__ __device__ constexpr int plan_rad(int N, int p) {
   }
   return 1;
 }
-__host__ __device__ constexpr int plan_np(int N) {
+__host__ __device__ constexpr int plan_np(int N, int V = 0) {
   int p = 0;
-  while (plan_rad(N, p) > 1) ++p;
+  while (plan_rad(N, p, V) > 1) ++p;
   return p;
 }
-__host__ __device__ constexpr int plan_ls(int N, int p) {
+__host__ __device__ constexpr int plan_ls(int N, int p, int V = 0) {
   int ls = 1;
-  for (int q = 0; q < p; ++q) ls *= plan_rad(N, q);
+  for (int q = 0; q < p; ++q) ls *= plan_rad(N, q, V);
   return ls;
 }
-__host__ __device__ constexpr int plan_T(int N) { return N / plan_rad(N, 0); }
-__host__ __device__ constexpr int plan_Q(int N, int p) {
-  return (N / plan_rad(N, p) + plan_T(N) - 1) / plan_T(N);
+__host__ __device__ constexpr int plan_T(int N, int V = 0) { return N / plan_rad(N, 0, V); }
+__host__ __device__ constexpr int plan_Q(int N, int p, int V = 0) {
+  return (N / plan_rad(N, p, V) + plan_T(N, V) - 1) / plan_T(N, V);
 }
 // registers per thread (complex values)
-__host__ __device__ constexpr int plan_E(int N) {
+__host__ __device__ constexpr int plan_E(int N, int V = 0) {
   int e = 0;
-  for (int p = 0; p < plan_np(N); ++p) {
-    const int v = plan_Q(N, p) * plan_rad(N, p);
+  for (int p = 0; p < plan_np(N, V); ++p) {
+    const int v = plan_Q(N, p, V) * plan_rad(N, p, V);
     if (v > e) e = v;
   }
   return e;
 }
-__host__ __device__ constexpr bool plan_ok(int N) {
+__host__ __device__ constexpr bool plan_ok(int N, int V = 0) {
   int prod = 1;
-  for (int p = 0; p < plan_np(N); ++p) prod *= plan_rad(N, p);
+  for (int p = 0; p < plan_np(N, V); ++p) prod *= plan_rad(N, p, V);
   return prod == N && N > 1;
 }
 
@@ -297,12 +327,12 @@ template <typename R, int r, bool INV> __host__ __device__ __forceinline__ void 
 // exit).  lds: this line's exchange buffer, element e at lds[e * ES].
 // Every per-pass quantity is a compile-time constant of PassC<N, P>.
 // ---------------------------------------------------------------------------
-template <int N, int P> struct PassC {
-  static constexpr int r = plan_rad(N, P);
+template <int N, int P, int V = 0> struct PassC {
+  static constexpr int r = plan_rad(N, P, V);
   static constexpr int NB = N / r;
-  static constexpr int LS = plan_ls(N, P);
-  static constexpr int Q = plan_Q(N, P);
-  static constexpr int T = plan_T(N);
+  static constexpr int LS = plan_ls(N, P, V);
+  static constexpr int Q = plan_Q(N, P, V);
+  static constexpr int T = plan_T(N, V);
   static constexpr bool FULL = (NB % T) == 0;
   static __host__ __device__ __forceinline__ int in_idx(int t, int q, int m) { return t + q * T + m * NB; }
   static __host__ __device__ __forceinline__ int out_idx(int t, int q, int m) {
@@ -311,17 +341,17 @@ template <int N, int P> struct PassC {
   }
   static __host__ __device__ __forceinline__ bool valid(int t, int q) { return FULL || t + q * T < NB; }
 };
-template <int N> struct RPlan {
-  static constexpr int NP = plan_np(N);
-  static constexpr int T = plan_T(N);
-  static constexpr int E = plan_E(N);
-  static_assert(plan_ok(N), "unsupported transform length");
+template <int N, int V = 0> struct RPlan {
+  static constexpr int NP = plan_np(N, V);
+  static constexpr int T = plan_T(N, V);
+  static constexpr int E = plan_E(N, V);
+  static_assert(plan_ok(N, V), "unsupported transform length");
 };
 
 // One pass's arithmetic on registers: twiddles (p > 0) then radix-r DFTs.
-template <typename R, int N, bool INV, int P>
+template <typename R, int N, bool INV, int P, int V = 0>
 __host__ __device__ __forceinline__ void rpass_compute(c2<R>* v, int t, bool act, const c2<R>* __restrict__ tw, int twm) {
-  using C = PassC<N, P>;
+  using C = PassC<N, P, V>;
   constexpr int r = C::r, NB = C::NB, LS = C::LS, Q = C::Q, T = C::T;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -342,9 +372,9 @@ __host__ __device__ __forceinline__ void rpass_compute(c2<R>* v, int t, bool act
 // LDS element stride: compile-time ES, or ES == 0 for a runtime stride es.
 template <int ES> __device__ __forceinline__ int lds_stride(int es) { return ES ? ES : es; }
 
-template <typename R, int N, int P, int ES>
+template <typename R, int N, int P, int ES, int V = 0>
 __device__ __forceinline__ void rpass_store(const c2<R>* v, int t, bool act, c2<R>* lds, int es) {
-  using C = PassC<N, P>;
+  using C = PassC<N, P, V>;
   const int st = lds_stride<ES>(es);
 #pragma unroll
   for (int q = 0; q < C::Q; ++q) {
@@ -353,9 +383,9 @@ __device__ __forceinline__ void rpass_store(const c2<R>* v, int t, bool act, c2<
     for (int m = 0; m < C::r; ++m) lds[C::out_idx(t, q, m) * st] = v[q * C::r + m];
   }
 }
-template <typename R, int N, int P, int ES>
+template <typename R, int N, int P, int ES, int V = 0>
 __device__ __forceinline__ void rpass_load(c2<R>* v, int t, bool act, const c2<R>* lds, int es) {
-  using C = PassC<N, P>;
+  using C = PassC<N, P, V>;
   const int st = lds_stride<ES>(es);
 #pragma unroll
   for (int q = 0; q < C::Q; ++q) {
@@ -366,52 +396,52 @@ __device__ __forceinline__ void rpass_load(c2<R>* v, int t, bool act, const c2<R
 }
 
 // Passes P.. of the transform; each exchange is bracketed by block barriers.
-template <typename R, int N, bool INV, int ES, int P>
+template <typename R, int N, bool INV, int ES, int P, int V = 0>
 __device__ __forceinline__ void rfft_from(c2<R>* v, int t, bool act, c2<R>* lds, const c2<R>* __restrict__ tw,
                                           int twm, int es) {
-  rpass_compute<R, N, INV, P>(v, t, act, tw, twm);
-  if constexpr (P + 1 < plan_np(N)) {
+  rpass_compute<R, N, INV, P, V>(v, t, act, tw, twm);
+  if constexpr (P + 1 < plan_np(N, V)) {
     __syncthreads();
-    rpass_store<R, N, P, ES>(v, t, act, lds, es);
+    rpass_store<R, N, P, ES, V>(v, t, act, lds, es);
     __syncthreads();
-    rpass_load<R, N, P + 1, ES>(v, t, act, lds, es);
-    rfft_from<R, N, INV, ES, P + 1>(v, t, act, lds, tw, twm, es);
+    rpass_load<R, N, P + 1, ES, V>(v, t, act, lds, es);
+    rfft_from<R, N, INV, ES, P + 1, V>(v, t, act, lds, tw, twm, es);
   }
 }
-// ES = 0: the LDS element stride is the runtime value es.
-template <typename R, int N, bool INV, int ES>
+// ES = 0: the LDS element stride is the runtime value es.  V: plan variant.
+template <typename R, int N, bool INV, int ES, int V = 0>
 __device__ __forceinline__ void rfft(c2<R>* v, int t, bool act, c2<R>* lds, const c2<R>* __restrict__ tw,
                                      int twm = 1, int es = 0) {
-  rfft_from<R, N, INV, ES, 0>(v, t, act, lds, tw, twm, es);
+  rfft_from<R, N, INV, ES, 0, V>(v, t, act, lds, tw, twm, es);
 }
 
 // Element index of register slot s after the transform (last pass outputs) and
 // before it (pass-0 inputs); s is a compile-time constant in unrolled loops.
-template <int N> using LastPass = PassC<N, plan_np(N) - 1>;
-template <int N> __host__ __device__ __forceinline__ int rfft_out_idx(int t, int s) {
-  return LastPass<N>::out_idx(t, s / LastPass<N>::r, s % LastPass<N>::r);
+template <int N, int V = 0> using LastPass = PassC<N, plan_np(N, V) - 1, V>;
+template <int N, int V = 0> __host__ __device__ __forceinline__ int rfft_out_idx(int t, int s) {
+  return LastPass<N, V>::out_idx(t, s / LastPass<N, V>::r, s % LastPass<N, V>::r);
 }
-template <int N> __host__ __device__ __forceinline__ int rfft_in_idx(int t, int s) {
-  return PassC<N, 0>::in_idx(t, s / PassC<N, 0>::r, s % PassC<N, 0>::r);
+template <int N, int V = 0> __host__ __device__ __forceinline__ int rfft_in_idx(int t, int s) {
+  return PassC<N, 0, V>::in_idx(t, s / PassC<N, 0, V>::r, s % PassC<N, 0, V>::r);
 }
-template <int N> __host__ __device__ __forceinline__ bool rfft_in_valid(int t, int s) {
-  return PassC<N, 0>::valid(t, s / PassC<N, 0>::r);
+template <int N, int V = 0> __host__ __device__ __forceinline__ bool rfft_in_valid(int t, int s) {
+  return PassC<N, 0, V>::valid(t, s / PassC<N, 0, V>::r);
 }
 // In the last pass LS = N / r = NB, so j < NB gives k = j and out_idx(t, q, m) =
 // j + m*NB = in_idx(t, q, m): each thread's outputs land on the elements it read
 // as that pass's inputs, and callers may write them back without a barrier.
-template <int N> constexpr bool last_pass_in_place() { return LastPass<N>::LS == LastPass<N>::NB; }
-template <int N> __host__ __device__ __forceinline__ bool rfft_out_valid(int t, int s) {
-  return LastPass<N>::valid(t, s / LastPass<N>::r);
+template <int N, int V = 0> constexpr bool last_pass_in_place() { return LastPass<N, V>::LS == LastPass<N, V>::NB; }
+template <int N, int V = 0> __host__ __device__ __forceinline__ bool rfft_out_valid(int t, int s) {
+  return LastPass<N, V>::valid(t, s / LastPass<N, V>::r);
 }
 // Number of register slots used at the input / output (variable templates, so
 // that loops over them have compile-time trip counts and fully unroll).
-template <int N> constexpr int kInSlots = PassC<N, 0>::Q * PassC<N, 0>::r;
-template <int N> constexpr int kOutSlots = LastPass<N>::Q * LastPass<N>::r;
+template <int N, int V = 0> constexpr int kInSlots = PassC<N, 0, V>::Q * PassC<N, 0, V>::r;
+template <int N, int V = 0> constexpr int kOutSlots = LastPass<N, V>::Q * LastPass<N, V>::r;
 // True when the thread's output element set equals its input element set (so a
 // forward transform's outputs can feed an inverse transform without exchange).
-template <int N> __host__ __device__ constexpr bool rfft_io_same() {
-  return plan_np(N) == 1 || (plan_np(N) == 2 && plan_rad(N, 0) == plan_rad(N, 1));
+template <int N, int V = 0> __host__ __device__ constexpr bool rfft_io_same() {
+  return plan_np(N, V) == 1 || (plan_np(N, V) == 2 && plan_rad(N, 0, V) == plan_rad(N, 1, V));
 }
 
 }  // namespace pf

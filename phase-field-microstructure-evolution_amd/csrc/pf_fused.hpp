@@ -478,8 +478,9 @@ __device__ __forceinline__ double recip_fast(double x) {
 }
 
 // Minimum waves per SIMD asked of the register allocator: FP32 fits 3 waves
-// (158 VGPRs for the plain update, 168 with the Parseval sums, no spills), FP64 2.
-template <typename R, bool PARSEVAL> constexpr int kZlineWaves = sizeof(R) == 4 ? 3 : 2;
+// (158 VGPRs for the plain update, 168 with the Parseval sums, no spills) with up
+// to 16 complex values per thread; plans holding more (E > 16) and FP64 ask for 2.
+template <typename R, bool PARSEVAL, int E = 16> constexpr int kZlineWaves = (sizeof(R) == 4 && E <= 16) ? 3 : 2;
 
 // The update coefficients of one parameter set, FP64.
 struct ZCo {
@@ -492,16 +493,16 @@ __device__ __forceinline__ ZCo zco(const UPar& u) {
 // BATCH (ROLE_REG only): the fields f0 .. f0 + nf - 1 run over every member in
 // mt, g.F per member, each with its own S, W, k = 0 modes and parameters; S, W
 // and dc64 are unused.
-template <typename R, int NZ, int ROLE, bool DC64, bool PARSEVAL, bool BATCH = false>
-__global__ void __launch_bounds__(col_tile(NZ, sizeof(c2<R>)) * plan_T(NZ))
-__attribute__((amdgpu_waves_per_eu(kZlineWaves<R, PARSEVAL>)))
+template <typename R, int NZ, int ROLE, bool DC64, bool PARSEVAL, bool BATCH = false, int V = 0>
+__global__ void __launch_bounds__(col_tile(NZ, sizeof(c2<R>)) * plan_T(NZ, V))
+__attribute__((amdgpu_waves_per_eu(kZlineWaves<R, PARSEVAL, plan_E(NZ, V)>)))
 k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
         double* __restrict__ dc64, double* __restrict__ part, const c2<R>* __restrict__ tw, int f0, int nf,
-        long tm, long tn, const Member<R>* __restrict__ mt) {
+        long tm, long tn, long run_tiles, const Member<R>* __restrict__ mt) {
   static_assert(!BATCH || ROLE == ROLE_REG, "batched members step without the mobility chain");
   constexpr int TX = col_tile(NZ, sizeof(c2<R>));
-  constexpr int NI = kInSlots<NZ>, NO = kOutSlots<NZ>, E = plan_E(NZ);
-  __shared__ c2<R> sh[(plan_np(NZ) > 1) ? NZ * TX : 1];
+  constexpr int NI = kInSlots<NZ, V>, NO = kOutSlots<NZ, V>, E = plan_E(NZ, V);
+  __shared__ c2<R> sh[(plan_np(NZ, V) > 1) ? NZ * TX : 1];
   __shared__ double red[16];
   const long tpf = tm + tn;
   const long total = (long)nf * tpf;
@@ -511,7 +512,7 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
   c2<R> v[E];
   c2<R> sv[NO];
   c2<R> xv[(ROLE == ROLE_CMOB) ? NO : 1];
-  const long tile0 = xcd_runs(blockIdx.x, gridDim.x, ((long)g.M + TX - 1) / TX);
+  const long tile0 = xcd_runs(blockIdx.x, gridDim.x, run_tiles);
   for (long tile = tile0; tile < total; tile += gridDim.x) {
     int f = f0 + (int)(tile / tpf);
     Bank<R> Sb = S, Wb = W;
@@ -540,12 +541,12 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
     // 1. all loads of the tile at once: transform input, spectrum (and f^) at the output rows
 #pragma unroll
     for (int s = 0; s < NI; ++s)
-      if (act && rfft_in_valid<NZ>(t, s)) v[s] = ld_nt<R>(ib + (long)rfft_in_idx<NZ>(t, s) * Wd);
+      if (act && rfft_in_valid<NZ, V>(t, s)) v[s] = ld_nt<R>(ib + (long)rfft_in_idx<NZ, V>(t, s) * Wd);
 #pragma unroll
     for (int s = 0; s < NO; ++s)
-      if (act && rfft_out_valid<NZ>(t, s)) {
-        sv[s] = ld_nt<R>(sb + (long)rfft_out_idx<NZ>(t, s) * Wd);
-        if constexpr (ROLE == ROLE_CMOB) xv[s] = xb[(long)rfft_out_idx<NZ>(t, s) * Wd];
+      if (act && rfft_out_valid<NZ, V>(t, s)) {
+        sv[s] = ld_nt<R>(sb + (long)rfft_out_idx<NZ, V>(t, s) * Wd);
+        if constexpr (ROLE == ROLE_CMOB) xv[s] = xb[(long)rfft_out_idx<NZ, V>(t, s) * Wd];
       }
     const long col = col0 + c;
     const int kx = isn ? g.M : (int)(col % g.M);
@@ -563,11 +564,11 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
     }
     const double kap = (f == 0) ? zc.kc : zc.kphi;
     // 2. forward z, update in registers
-    rfft<R, NZ, false, TX>(v, t, act, sh + c, tw);
+    rfft<R, NZ, false, TX, V>(v, t, act, sh + c, tw);
 #pragma unroll
     for (int s = 0; s < NO; ++s) {
-      if (!(act && rfft_out_valid<NZ>(t, s))) continue;
-      const int e = rfft_out_idx<NZ>(t, s);
+      if (!(act && rfft_out_valid<NZ, V>(t, s))) continue;
+      const int e = rfft_out_idx<NZ, V>(t, s);
       const KCoefR<double> k = kcoef_col<double>(ktz, kc, e);
       const c2<R> w = v[s];
       const long ga = (long)e * Wd;
@@ -622,9 +623,9 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
       R pa1 = R(0), pa2 = R(0);
 #pragma unroll
       for (int s = 0; s < NO; ++s) {
-        if (!(act && rfft_out_valid<NZ>(t, s))) continue;
+        if (!(act && rfft_out_valid<NZ, V>(t, s))) continue;
         const R m2 = v[s].x * v[s].x + v[s].y * v[s].y;
-        const int e = rfft_out_idx<NZ>(t, s);
+        const int e = rfft_out_idx<NZ, V>(t, s);
         const R zp = (2 * e == NZ) ? R(0) : (R)ktz.az[e];  // a'_z: a_z zeroed at Nyquist
         pa1 += m2;
         pa2 += zp * zp * m2;
@@ -633,21 +634,21 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
       acc += (0.5 * kap) * wgt * (kc.kp2 * (double)pa1 + (double)pa2);
     }
     // 3. inverse z (exchange first when the output rows are not the input rows)
-    if constexpr (!rfft_io_same<NZ>()) {
+    if constexpr (!rfft_io_same<NZ, V>()) {
       __syncthreads();
 #pragma unroll
       for (int s = 0; s < NO; ++s)
-        if (act && rfft_out_valid<NZ>(t, s)) sh[rfft_out_idx<NZ>(t, s) * TX + c] = v[s];
+        if (act && rfft_out_valid<NZ, V>(t, s)) sh[rfft_out_idx<NZ, V>(t, s) * TX + c] = v[s];
       __syncthreads();
 #pragma unroll
       for (int s = 0; s < NI; ++s)
-        if (act && rfft_in_valid<NZ>(t, s)) v[s] = sh[rfft_in_idx<NZ>(t, s) * TX + c];
+        if (act && rfft_in_valid<NZ, V>(t, s)) v[s] = sh[rfft_in_idx<NZ, V>(t, s) * TX + c];
     }
-    rfft<R, NZ, true, TX>(v, t, act, sh + c, tw);
+    rfft<R, NZ, true, TX, V>(v, t, act, sh + c, tw);
     c2<R>* const ob = (ROLE == ROLE_GHAT) ? (isn ? Y.nyq : Y.main) + off0 : wb;
 #pragma unroll
     for (int s = 0; s < NO; ++s)
-      if (act && rfft_out_valid<NZ>(t, s)) st_nt<R>(ob + (long)rfft_out_idx<NZ>(t, s) * Wd, v[s]);
+      if (act && rfft_out_valid<NZ, V>(t, s)) st_nt<R>(ob + (long)rfft_out_idx<NZ, V>(t, s) * Wd, v[s]);
     __syncthreads();
   }
   if (PARSEVAL) {

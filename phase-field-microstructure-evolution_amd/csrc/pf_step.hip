@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -33,6 +34,9 @@ constexpr long kMaxGrid = 8192;
 // more resident tiles per CU keep more rows in flight (measured 2.5% faster
 // at 32768 than at 8192 blocks, 256^3 x 17 fields).
 constexpr long kZlineMaxGrid = 32768;
+// Largest grids a tuning may ask for (the block-partial buffers are sized for them).
+constexpr long kXlineGridCap = 16384;
+constexpr long kZlineGridCap = 65536;
 
 using UCoef = UPar;
 
@@ -554,10 +558,17 @@ struct pf_handle {
 
   double* staging = nullptr;  // (var+1)*P doubles for state I/O
 
+  // launch shapes of the fused kernels (pf_set_tuning; from the tuned table at
+  // pf_create): x-line lines per block (0 = automatic) and grid, z-line grid
+  // and XCD run length in main-block rows
+  pf_tuning tune{0, 0, 0, -1, 0};
+
   std::vector<void*> allocs;
 
   // timing
   bool timing = false;
+  unsigned timing_mask = ~0u;  // stages timed while timing is on
+  bool ev_open = false;        // tbeg recorded a start event that tend must close
   std::vector<hipEvent_t> ev;
   std::vector<int> ev_stage;
   size_t ev_used = 0;
@@ -608,6 +619,8 @@ template <typename R> Bank<R> sub(const Bank<R>& b, const Geo& g, int f) {
 // ---- timers ----
 void tbeg(pf_handle* h, int stage) {
   if (!h->timing) return;
+  h->ev_open = (h->timing_mask >> stage) & 1;
+  if (!h->ev_open) return;
   if (h->ev_used + 2 > h->ev.size()) {
     for (int i = 0; i < 64; ++i) {
       hipEvent_t e;
@@ -620,7 +633,8 @@ void tbeg(pf_handle* h, int stage) {
   (void)hipEventRecord(h->ev[h->ev_used], h->s);
 }
 void tend(pf_handle* h) {
-  if (!h->timing) return;
+  if (!h->timing || !h->ev_open) return;
+  h->ev_open = false;
   (void)hipEventRecord(h->ev[h->ev_used + 1], h->s);
   h->ev_used += 2;
 }
@@ -819,10 +833,11 @@ int launch_xline_n(pf_handle* h, c2<R>* wm, c2<R>* wn, R* mobd, double* part, lo
                    bool mob, long nt, size_t shb) {
   const Geo& g = h->g;
   const long ngroups = g.plane / LINES;
-  long grid = std::min<long>(ngroups, kMaxGrid);
+  const long cap = (!XONLY && h->tune.xline_grid > 0) ? h->tune.xline_grid : kMaxGrid;
+  long grid = std::min<long>(ngroups, cap);
   if (grid >= 8) grid -= grid % 8;
   const c2<R>* tw = twiddle_table<R>(NX);
-  if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");
+  if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");
   if (shb > 65536 &&
       hipFuncSetAttribute(reinterpret_cast<const void*>(&k_xline<R, NX, LINES, XONLY>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shb) != hipSuccess)
@@ -848,12 +863,14 @@ int launch_xline(pf_handle* h, c2<R>* wm, c2<R>* wn, R* mobd, double* part, long
     const long T = plan_T(N / 2);                                                                        \
     int want = 1;                                                                                        \
     while (want < 8 && F * want * T < kXlineMinThreads) want *= 2;                                       \
+    const int tl = XONLY ? 0 : h->tune.xline_lines;                                                      \
+    if (tl > 0) want = tl;                                                                               \
     if (want >= 8) { PF_TRY(N, 8) }                                                                      \
-    if (want >= 4) { PF_TRY(N, 4) }                                                                      \
-    if (want >= 2) { PF_TRY(N, 2) }                                                                      \
-    PF_TRY(N, 1)                                                                                         \
-    return fail(PF_ENOTIMPL, "x-line kernel: " + std::to_string(F) +                                     \
-                                 " fields of length " #N " exceed one block's threads or LDS"); \
+    if (want >= 4 && want < 8) { PF_TRY(N, 4) }                                                          \
+    if (want >= 2 && want < 4) { PF_TRY(N, 2) }                                                          \
+    if (tl == 0 || want == 1) { PF_TRY(N, 1) }                                                           \
+    return fail(PF_ENOTIMPL, "x-line kernel: " + std::to_string(F) + " fields of length " #N " in " +    \
+                                 std::to_string(want) + "-line blocks exceed one block's threads or LDS"); \
   }
     PF_GRID_SIZES(PF_CASE)
 #undef PF_CASE
@@ -922,6 +939,13 @@ int launch_ypass(pf_handle* h, c2<R>* main, c2<R>* nyq, int nf) {
   }
 }
 
+// The z-line's plan variant: the tuned one for the FP32 update of every field
+// (ROLE_REG), the default for FP64 and for the mobility chain's roles.
+template <typename R, int ROLE> int zline_plan_of(const pf_handle* h, int n) {
+  if (sizeof(R) != 4 || ROLE != ROLE_REG || h->tune.zline_plan <= 0) return 0;
+  return plan_alt(n, h->tune.zline_plan, 0) != 0 ? h->tune.zline_plan : 0;
+}
+
 template <typename R, int ROLE, bool DC64, bool PARSEVAL>
 int launch_zline(pf_handle* h, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y, double* part, int f0, int nf,
                  long* grid_out) {
@@ -931,15 +955,34 @@ int launch_zline(pf_handle* h, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y, doubl
   case N: {                                                                                          \
     constexpr int TX = col_tile(N, sizeof(c2<R>));                                                   \
     const long tm = ((long)g.M * g.ny + TX - 1) / TX, tn = ((long)g.ny + TX - 1) / TX;               \
-    long grid = std::min<long>((tm + tn) * nf, kZlineMaxGrid);                                      \
-    /* whole XCD runs of one main-block row (xcd_runs): extra blocks find no tile */                 \
-    const long run = 8 * (((long)g.M + TX - 1) / TX);                                                \
-    if ((grid + run - 1) / run * run <= kZlineMaxGrid) grid = (grid + run - 1) / run * run;         \
+    const long cap = h->tune.zline_grid > 0 ? h->tune.zline_grid : kZlineMaxGrid;                    \
+    long grid = std::min<long>((tm + tn) * nf, cap);                                                 \
+    /* whole XCD runs of main-block rows (xcd_runs): extra blocks find no tile */                    \
+    const long rows = h->tune.zline_run_rows < 0 ? 1 : h->tune.zline_run_rows;                      \
+    const long runt = rows * (((long)g.M + TX - 1) / TX), run = 8 * runt;                           \
+    if (run > 0 && (grid + run - 1) / run * run <= cap) grid = (grid + run - 1) / run * run;        \
     const c2<R>* tw = twiddle_table<R>(N);                                                           \
-    if (!tw) return fail(PF_EHIP, "twiddle table allocation failed");                                \
+    if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");                                    \
+    const int plan = zline_plan_of<R, ROLE>(h, N);                                                   \
+    if constexpr (sizeof(R) == 4 && ROLE == ROLE_REG && plan_alt(N, 1, 0) != 0) {                    \
+      if (plan == 1) {                                                                               \
+        hipLaunchKernelGGL((k_zline<R, N, ROLE, DC64, PARSEVAL, false, 1>), dim3((unsigned)grid),    \
+                           dim3(TX * plan_T(N, 1)), 0, h->s, g, h->u, h->kt, S, W, X, Y, h->dc64, part, \
+                           tw, f0, nf, tm, tn, runt, (const Member<R>*)nullptr);                     \
+        if (grid_out) *grid_out = grid;                                                              \
+        return 0;                                                                                    \
+      }                                                                                              \
+      if (plan == 2) {                                                                               \
+        hipLaunchKernelGGL((k_zline<R, N, ROLE, DC64, PARSEVAL, false, 2>), dim3((unsigned)grid),    \
+                           dim3(TX * plan_T(N, 2)), 0, h->s, g, h->u, h->kt, S, W, X, Y, h->dc64, part, \
+                           tw, f0, nf, tm, tn, runt, (const Member<R>*)nullptr);                     \
+        if (grid_out) *grid_out = grid;                                                              \
+        return 0;                                                                                    \
+      }                                                                                              \
+    }                                                                                                \
     hipLaunchKernelGGL((k_zline<R, N, ROLE, DC64, PARSEVAL>), dim3((unsigned)grid),                  \
                        dim3(TX * plan_T(N)), 0, h->s, g, h->u, h->kt, S, W, X, Y, h->dc64, part, tw,      \
-                       f0, nf, tm, tn, (const Member<R>*)nullptr);                                   \
+                       f0, nf, tm, tn, runt, (const Member<R>*)nullptr);                             \
     if (grid_out) *grid_out = grid;                                                                  \
     return 0;                                                                                        \
   }
@@ -956,14 +999,16 @@ int launch_zline(pf_handle* h, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y, doubl
 // arithmetic are those of a single member's step, so each member's result is
 // bit-identical to stepping it alone.
 // ---------------------------------------------------------------------------
-int xline_batch(const Geo& g, const UPar& u, const Member<float>* mt, long nmem, hipStream_t s) {
+int xline_batch(const Geo& g, const UPar& u, const pf_tuning& tn, const Member<float>* mt, long nmem,
+                hipStream_t s) {
   long nt = 0;
   size_t shb = 0;
+  const long cap = tn.xline_grid > 0 ? tn.xline_grid : kMaxGrid;
   switch (g.nx) {
 #define PF_TRY(N, L)                                                                                   \
   if (xline_fits<float, N, L, false>(g, &nt, &shb)) {                                                  \
     const long ngroups = nmem * (g.plane / L);                                                         \
-    long grid = std::min<long>(ngroups, kMaxGrid);                                                     \
+    long grid = std::min<long>(ngroups, cap);                                                          \
     if (grid >= 8) grid -= grid % 8;                                                                   \
     const c2<float>* tw = twiddle_table<float>(N);                                                     \
     if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");                                      \
@@ -981,10 +1026,11 @@ int xline_batch(const Geo& g, const UPar& u, const Member<float>* mt, long nmem,
     const long T = plan_T(N / 2);                                                                      \
     int want = 1;                                                                                      \
     while (want < 8 && g.F * want * T < kXlineMinThreads) want *= 2;                                   \
+    if (tn.xline_lines > 0) want = tn.xline_lines;                                                     \
     if (want >= 8) { PF_TRY(N, 8) }                                                                    \
-    if (want >= 4) { PF_TRY(N, 4) }                                                                    \
-    if (want >= 2) { PF_TRY(N, 2) }                                                                    \
-    PF_TRY(N, 1)                                                                                       \
+    if (want >= 4 && want < 8) { PF_TRY(N, 4) }                                                        \
+    if (want >= 2 && want < 4) { PF_TRY(N, 2) }                                                        \
+    if (tn.xline_lines == 0 || want == 1) { PF_TRY(N, 1) }                                             \
     return fail(PF_ENOTIMPL, "x-line kernel: " + std::to_string(g.F) +                                \
                                  " fields of length " #N " exceed one block's threads or LDS");        \
   }
@@ -1015,22 +1061,40 @@ template <bool INV> int ypass_batch(const Geo& g, const Member<float>* mt, long 
   }
 }
 
-int zline_batch(const Geo& g, const UPar& u, const KTab& kt, const Member<float>* mt, long nmem, hipStream_t s) {
+int zline_batch(const Geo& g, const UPar& u, const pf_tuning& tun, const KTab& kt, const Member<float>* mt,
+                long nmem, hipStream_t s) {
   const Bank<float> none{nullptr, nullptr};
+  const long cap = tun.zline_grid > 0 ? tun.zline_grid : kZlineMaxGrid;
+  const long rows = tun.zline_run_rows < 0 ? 1 : tun.zline_run_rows;
   switch (g.nz) {
 #define PF_CASE(N)                                                                                   \
   case N: {                                                                                          \
     constexpr int TX = col_tile(N, sizeof(c2<float>));                                               \
     const long tm = ((long)g.M * g.ny + TX - 1) / TX, tn = ((long)g.ny + TX - 1) / TX;               \
     const long nf = nmem * g.F;                                                                      \
-    long grid = std::min<long>((tm + tn) * nf, kZlineMaxGrid);                                       \
-    const long run = 8 * (((long)g.M + TX - 1) / TX);                                                \
-    if ((grid + run - 1) / run * run <= kZlineMaxGrid) grid = (grid + run - 1) / run * run;         \
+    long grid = std::min<long>((tm + tn) * nf, cap);                                                 \
+    const long runt = rows * (((long)g.M + TX - 1) / TX), run = 8 * runt;                            \
+    if (run > 0 && (grid + run - 1) / run * run <= cap) grid = (grid + run - 1) / run * run;        \
     const c2<float>* tw = twiddle_table<float>(N);                                                   \
     if (!tw) return fail(PF_EHIP, "twiddle table set-up failed");                                    \
+    const int plan = (tun.zline_plan > 0 && plan_alt(N, tun.zline_plan, 0) != 0) ? tun.zline_plan : 0; \
+    if constexpr (plan_alt(N, 1, 0) != 0) {                                                          \
+      if (plan == 1) {                                                                               \
+        hipLaunchKernelGGL((k_zline<float, N, ROLE_REG, true, false, true, 1>), dim3((unsigned)grid), \
+                           dim3(TX * plan_T(N, 1)), 0, s, g, u, kt, none, none, none, none,          \
+                           (double*)nullptr, (double*)nullptr, tw, 0, (int)nf, tm, tn, runt, mt);     \
+        return 0;                                                                                    \
+      }                                                                                              \
+      if (plan == 2) {                                                                               \
+        hipLaunchKernelGGL((k_zline<float, N, ROLE_REG, true, false, true, 2>), dim3((unsigned)grid), \
+                           dim3(TX * plan_T(N, 2)), 0, s, g, u, kt, none, none, none, none,          \
+                           (double*)nullptr, (double*)nullptr, tw, 0, (int)nf, tm, tn, runt, mt);     \
+        return 0;                                                                                    \
+      }                                                                                              \
+    }                                                                                                \
     hipLaunchKernelGGL((k_zline<float, N, ROLE_REG, true, false, true>), dim3((unsigned)grid),       \
                        dim3(TX * plan_T(N)), 0, s, g, u, kt, none, none, none, none, (double*)nullptr, \
-                       (double*)nullptr, tw, 0, (int)nf, tm, tn, mt);                                \
+                       (double*)nullptr, tw, 0, (int)nf, tm, tn, runt, mt);                          \
     return 0;                                                                                        \
   }
     PF_GRID_SIZES(PF_CASE)
@@ -1194,6 +1258,71 @@ template <typename R> int put_reduced(pf_handle* h, const double* phi, const dou
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Tuned launch shapes: entries written by the tuner (autotune.py) from timings
+// on an MI355X, one per (grid, var, precision); pf_create takes the matching
+// entry, else the defaults.
+// ---------------------------------------------------------------------------
+struct TunedShape {
+  int nx, ny, nz, var, prec;  // prec: PF_PREC_FP64 / PF_PREC_FP32
+  pf_tuning t;
+};
+const TunedShape kTuned[] = {
+#include "pf_tuned.inc"
+    {0, 0, 0, 0, -1, {0, 0, 0, -1, 0}},
+};
+
+// Whether the x-line can run blocks of `lines` lines for this handle.
+bool xline_lines_fit(const pf_handle* h, int lines) {
+  const Geo& g = h->g;
+  long nt = 0;
+  size_t shb = 0;
+  const bool f32 = h->mode == PF_MODE_FP32_MIXED;
+  switch (g.nx) {
+#define PF_CASE(N)                                                                                      \
+  case N:                                                                                               \
+    switch (lines) {                                                                                    \
+      case 1: return f32 ? xline_fits<float, N, 1, false>(g, &nt, &shb) : xline_fits<double, N, 1, false>(g, &nt, &shb); \
+      case 2: return f32 ? xline_fits<float, N, 2, false>(g, &nt, &shb) : xline_fits<double, N, 2, false>(g, &nt, &shb); \
+      case 4: return f32 ? xline_fits<float, N, 4, false>(g, &nt, &shb) : xline_fits<double, N, 4, false>(g, &nt, &shb); \
+      case 8: return f32 ? xline_fits<float, N, 8, false>(g, &nt, &shb) : xline_fits<double, N, 8, false>(g, &nt, &shb); \
+      default: return false;                                                                            \
+    }
+    PF_GRID_SIZES(PF_CASE)
+#undef PF_CASE
+    default: return false;
+  }
+}
+
+int check_tuning(const pf_handle* h, const pf_tuning& t) {
+  if (!(t.xline_lines == 0 || t.xline_lines == 1 || t.xline_lines == 2 || t.xline_lines == 4 || t.xline_lines == 8))
+    return fail(PF_EINVAL, "xline_lines must be 0, 1, 2, 4 or 8");
+  if (t.xline_lines > 0 && !xline_lines_fit(h, t.xline_lines))
+    return fail(PF_EINVAL, "x-line blocks of " + std::to_string(t.xline_lines) + " lines do not fit this grid");
+  if (t.xline_grid != 0 && (t.xline_grid < 256 || t.xline_grid > kXlineGridCap))
+    return fail(PF_EINVAL, "xline_grid must be 0 or in [256, " + std::to_string(kXlineGridCap) + "]");
+  if (t.zline_grid != 0 && (t.zline_grid < 1024 || t.zline_grid > kZlineGridCap))
+    return fail(PF_EINVAL, "zline_grid must be 0 or in [1024, " + std::to_string(kZlineGridCap) + "]");
+  if (t.zline_run_rows < -1 || t.zline_run_rows > 8) return fail(PF_EINVAL, "zline_run_rows must be in [-1, 8]");
+  if (t.zline_plan < 0 || t.zline_plan > 2) return fail(PF_EINVAL, "zline_plan must be 0, 1 or 2");
+  if (t.zline_plan > 0 && (h->mode != PF_MODE_FP32_MIXED || plan_alt(h->g.nz, t.zline_plan, 0) == 0))
+    return fail(PF_EINVAL, "no z-line plan variant " + std::to_string(t.zline_plan) + " for nz = " +
+                               std::to_string(h->g.nz) + " in this form (variants exist for FP32-mixed at "
+                               "nz = 64, 128, 160, 192, 256, 512)");
+  return 0;
+}
+
+void apply_tuned_table(pf_handle* h) {
+  const char* env = std::getenv("PF_TUNING");
+  if (env && std::string(env) == "default") return;
+  const int prec = h->mode == PF_MODE_FP32_MIXED ? PF_PREC_FP32 : PF_PREC_FP64;
+  for (const TunedShape& e : kTuned)
+    if (e.nx == h->g.nx && e.ny == h->g.ny && e.nz == h->g.nz && e.var == h->g.var && e.prec == prec) {
+      if (check_tuning(h, e.t) == 0) h->tune = e.t;
+      return;
+    }
+}
+
 // ===========================================================================
 // C-ABI
 // ===========================================================================
@@ -1263,18 +1392,36 @@ extern "C" int pf_create(const pf_params* p, int nx, int ny, int nz, int var, in
                                         std::to_string(nx) + "-point x lines exceed one block's threads or LDS"));
     if ((rc = alloc_reduced<float>(h))) return bail(rc);
   }
-  if ((rc = dalloc(h, (void**)&h->part_nl, sizeof(double) * NPART_NL * kMaxGrid))) return bail(rc);
-  if ((rc = dalloc(h, (void**)&h->part_fi, sizeof(double) * 2 * std::max(kMaxGrid, kZlineMaxGrid)))) return bail(rc);
+  if ((rc = dalloc(h, (void**)&h->part_nl, sizeof(double) * NPART_NL * std::max(kMaxGrid, kXlineGridCap))))
+    return bail(rc);
+  if ((rc = dalloc(h, (void**)&h->part_fi, sizeof(double) * 2 * std::max(kMaxGrid, kZlineGridCap)))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->staging, sizeof(double) * (size_t)g.P * g.F))) return bail(rc);
   h->n_part_nl = grid_for(g.P, NB);
   if ((rc = ensure_diag(h, 64))) return bail(rc);
   if ((rc = dalloc(h, (void**)&h->dctr, sizeof(int)))) return bail(rc);
-  // the transforms' twiddle tables are made now: a step under graph capture
-  // must not allocate
+  // the twiddle tables are set up now (their first use on a device copies
+  // them), not inside a step that a caller may capture
   for (int n : {nx, ny, nz})
     if (!twiddle_table<float>(n) || !twiddle_table<double>(n))
-      return bail(fail(PF_EHIP, "twiddle table allocation failed"));
+      return bail(fail(PF_EHIP, "twiddle table set-up failed"));
+  if (mode != PF_MODE_FP64_FAITHFUL) apply_tuned_table(h);
   *out = h;
+  return 0;
+}
+
+extern "C" int pf_set_tuning(pf_handle* h, const pf_tuning* t) {
+  if (!h || !t) return fail(PF_EINVAL, "null argument");
+  DeviceGuard dg(h->dev);
+  const int rc = check_tuning(h, *t);
+  if (rc) return rc;
+  drop_graph(h);  // a captured period holds the old launch shapes
+  h->tune = *t;
+  return 0;
+}
+
+extern "C" int pf_get_tuning(const pf_handle* h, pf_tuning* t) {
+  if (!h || !t) return fail(PF_EINVAL, "null argument");
+  *t = h->tune;
   return 0;
 }
 
@@ -1481,7 +1628,10 @@ bool batchable(const pf_handle* h) {
   return h->mode == PF_MODE_FP32_MIXED && h->p.D_gb == 0.0 && !h->timing;
 }
 bool same_shape(const pf_handle* a, const pf_handle* b) {
-  return a->g.nx == b->g.nx && a->g.ny == b->g.ny && a->g.nz == b->g.nz && a->g.var == b->g.var;
+  const pf_tuning &ta = a->tune, &tb = b->tune;
+  return a->g.nx == b->g.nx && a->g.ny == b->g.ny && a->g.nz == b->g.nz && a->g.var == b->g.var &&
+         ta.xline_lines == tb.xline_lines && ta.xline_grid == tb.xline_grid && ta.zline_grid == tb.zline_grid &&
+         ta.zline_run_rows == tb.zline_run_rows && ta.zline_plan == tb.zline_plan;
 }
 
 // A group of batchable members of one shape: its device member table, stepped
@@ -1512,9 +1662,9 @@ int step_group(const Group& gr) {
   pf_handle* h0 = gr.mem[0];
   const Geo& g = h0->g;
   const long nmem = (long)gr.mem.size();
-  int rc = xline_batch(g, h0->u, gr.table, nmem, h0->s);
+  int rc = xline_batch(g, h0->u, h0->tune, gr.table, nmem, h0->s);
   if (!rc) rc = ypass_batch<false>(g, gr.table, nmem, h0->s);
-  if (!rc) rc = zline_batch(g, h0->u, h0->kt, gr.table, nmem, h0->s);
+  if (!rc) rc = zline_batch(g, h0->u, h0->tune, h0->kt, gr.table, nmem, h0->s);
   if (!rc) rc = ypass_batch<true>(g, gr.table, nmem, h0->s);
   return rc;
 }
@@ -1728,6 +1878,16 @@ extern "C" int pf_diagnose(pf_handle* h, pf_diag* out) {
 extern "C" int pf_set_timing(pf_handle* h, int on) {
   if (!h) return fail(PF_EINVAL, "null handle");
   h->timing = on != 0;
+  h->timing_mask = ~0u;
+  for (int i = 0; i < NSTAGE; ++i) { h->st_ms[i] = 0.0; h->st_cnt[i] = 0; }
+  return 0;
+}
+
+extern "C" int pf_set_timing_stages(pf_handle* h, unsigned mask) {
+  if (!h) return fail(PF_EINVAL, "null handle");
+  if (mask == 0) return fail(PF_EINVAL, "empty stage mask");
+  h->timing = true;
+  h->timing_mask = mask;
   for (int i = 0; i < NSTAGE; ++i) { h->st_ms[i] = 0.0; h->st_cnt[i] = 0; }
   return 0;
 }

@@ -91,9 +91,8 @@ def run_ensemble(pf, members, nsteps: int, mode: str = "fp32_mixed", seeds: int 
     sims = []
     for m in members:
         g = m["grid"]
-        phi, con = pf.ic_voronoi(g, g, g, m["var"], seeds, -(m["member"] + 1), 0.4, 0.02)
         s = pf.Simulation(m["params"], g, g, g, m["var"], mode)
-        s.put_state(phi, con)
+        s.put_voronoi(seeds, -(m["member"] + 1), 0.4, 0.02)  # = ic_voronoi + put_state, built on the device
         sims.append(s)
     step = chunk if chunk > 0 else nsteps
     done = 0

@@ -1,7 +1,8 @@
 // CPU emulation of the register-resident Stockham schedule in pf_fft2.hpp:
 // every thread of a line runs each pass's arithmetic on its registers, and the
 // LDS exchanges between passes are replayed sequentially.  Output: one line per
-// (N, direction) with the max error against a brute-force long-double DFT.
+// (N, plan variant) with the max error of both directions against a
+// brute-force long-double DFT.
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -13,27 +14,27 @@
 using namespace pf;
 typedef std::complex<long double> cl;
 
-template <int N, bool INV, int P>
+template <int N, bool INV, int P, int V>
 void passes(std::vector<std::vector<double2>>& regs, const double2* tw) {
-  constexpr int T = plan_T(N);
-  for (int t = 0; t < T; ++t) rpass_compute<double, N, INV, P>(regs[t].data(), t, true, tw, 1);
-  if constexpr (P + 1 < plan_np(N)) {
+  constexpr int T = plan_T(N, V);
+  for (int t = 0; t < T; ++t) rpass_compute<double, N, INV, P, V>(regs[t].data(), t, true, tw, 1);
+  if constexpr (P + 1 < plan_np(N, V)) {
     std::vector<double2> lds(N);
-    constexpr int r = plan_rad(N, P), Q = plan_Q(N, P);
+    using C0 = PassC<N, P, V>;
+    using C1 = PassC<N, P + 1, V>;
     for (int t = 0; t < T; ++t)
-      for (int q = 0; q < Q; ++q)
-        if (PassC<N, P>::valid(t, q))
-          for (int m = 0; m < r; ++m) lds[PassC<N, P>::out_idx(t, q, m)] = regs[t][q * r + m];
-    constexpr int r1 = plan_rad(N, P + 1), Q1 = plan_Q(N, P + 1);
+      for (int q = 0; q < C0::Q; ++q)
+        if (C0::valid(t, q))
+          for (int m = 0; m < C0::r; ++m) lds[C0::out_idx(t, q, m)] = regs[t][q * C0::r + m];
     for (int t = 0; t < T; ++t)
-      for (int q = 0; q < Q1; ++q)
-        if (PassC<N, P + 1>::valid(t, q))
-          for (int m = 0; m < r1; ++m) regs[t][q * r1 + m] = lds[PassC<N, P + 1>::in_idx(t, q, m)];
-    passes<N, INV, P + 1>(regs, tw);
+      for (int q = 0; q < C1::Q; ++q)
+        if (C1::valid(t, q))
+          for (int m = 0; m < C1::r; ++m) regs[t][q * C1::r + m] = lds[C1::in_idx(t, q, m)];
+    passes<N, INV, P + 1, V>(regs, tw);
   }
 }
 
-template <int N, bool INV> double check() {
+template <int N, bool INV, int V> double check() {
   std::vector<double> re(N), im(N);
   host_twiddles(N, re.data(), im.data());
   std::vector<double2> tw(N);
@@ -44,17 +45,18 @@ template <int N, bool INV> double check() {
     s = s * 1103515245u + 12345u; x[i].x = ((s >> 8) % 20001) / 10000.0 - 1.0;
     s = s * 1103515245u + 12345u; x[i].y = ((s >> 8) % 20001) / 10000.0 - 1.0;
   }
-  constexpr int T = plan_T(N), E = plan_E(N);
+  constexpr int T = plan_T(N, V), E = plan_E(N, V);
+  static_assert(plan_ok(N, V), "plan does not factor N");
   std::vector<std::vector<double2>> regs(T, std::vector<double2>(E));
   for (int t = 0; t < T; ++t)
-    for (int sl = 0; sl < kInSlots<N>; ++sl)
-      if (rfft_in_valid<N>(t, sl)) regs[t][sl] = x[rfft_in_idx<N>(t, sl)];
-  passes<N, INV, 0>(regs, tw.data());
+    for (int sl = 0; sl < kInSlots<N, V>; ++sl)
+      if (rfft_in_valid<N, V>(t, sl)) regs[t][sl] = x[rfft_in_idx<N, V>(t, sl)];
+  passes<N, INV, 0, V>(regs, tw.data());
   std::vector<double2> y(N);
   std::vector<int> seen(N, 0);
   for (int t = 0; t < T; ++t)
-    for (int sl = 0; sl < kOutSlots<N>; ++sl)
-      if (rfft_out_valid<N>(t, sl)) { y[rfft_out_idx<N>(t, sl)] = regs[t][sl]; seen[rfft_out_idx<N>(t, sl)]++; }
+    for (int sl = 0; sl < kOutSlots<N, V>; ++sl)
+      if (rfft_out_valid<N, V>(t, sl)) { y[rfft_out_idx<N, V>(t, sl)] = regs[t][sl]; seen[rfft_out_idx<N, V>(t, sl)]++; }
   double err = 0.0;
   const long double pi2 = 6.283185307179586476925286766559L;
   for (int k = 0; k < N; ++k) {
@@ -73,22 +75,27 @@ template <int N, bool INV> double check() {
 // 1 when, for every thread and valid slot of the last pass, the output element
 // is the element the thread loaded as that pass's input (what lets callers
 // write the transform back into its exchange buffer without a barrier).
-template <int N> int last_pass_in_place_emul() {
-  using C = LastPass<N>;
+template <int N, int V> int last_pass_in_place_emul() {
+  using C = LastPass<N, V>;
   for (int t = 0; t < C::T; ++t)
     for (int q = 0; q < C::Q; ++q) {
       if (!C::valid(t, q)) continue;
       for (int m = 0; m < C::r; ++m)
         if (C::out_idx(t, q, m) != C::in_idx(t, q, m)) return 0;
     }
-  return last_pass_in_place<N>() ? 1 : 0;
+  return last_pass_in_place<N, V>() ? 1 : 0;
 }
 
-#define RUN(N) \
-  std::printf("%d %d %.3e %.3e %d\n", N, plan_np(N), check<N, false>(), check<N, true>(), \
-              last_pass_in_place_emul<N>());
+// columns: N, plan variant, passes, forward error, inverse error, in-place
+// last pass, threads per line
+#define RUNV(N, V)                                                                                   \
+  std::printf("%d %d %d %.3e %.3e %d %d\n", N, V, plan_np(N, V), check<N, false, V>(), check<N, true, V>(), \
+              last_pass_in_place_emul<N, V>(), plan_T(N, V));
+#define RUN(N) RUNV(N, 0)
 int main() {
   RUN(2) RUN(3) RUN(4) RUN(5) RUN(6) RUN(8) RUN(10) RUN(12) RUN(16) RUN(20) RUN(24) RUN(32) RUN(40)
   RUN(48) RUN(64) RUN(80) RUN(96) RUN(128) RUN(160) RUN(192) RUN(256) RUN(512)
+  RUNV(64, 1) RUNV(128, 1) RUNV(160, 1) RUNV(192, 1) RUNV(256, 1) RUNV(512, 1)
+  RUNV(64, 2) RUNV(128, 2) RUNV(160, 2) RUNV(192, 2) RUNV(256, 2) RUNV(512, 2)
   return 0;
 }

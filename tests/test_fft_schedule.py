@@ -24,18 +24,26 @@ def emul_output(tmp_path_factory):
            os.path.join(ROOT, "tests", "cpu", "fft2_emul.cpp"), "-o", exe]
     subprocess.run(cmd, check=True, capture_output=True, timeout=600)
     out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=300).stdout
-    return {int(l.split()[0]): l.split() for l in out.strip().splitlines()}
+    return {(int(l.split()[0]), int(l.split()[1])): l.split() for l in out.strip().splitlines()}
 
 
 LENGTHS = [2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 512]
+# alternative radix sequences (plan variants 1, 2), candidates of the tuner
+VARIANTS = [(n, v) for v in (1, 2) for n in (64, 128, 160, 192, 256, 512)]
 
 
-@pytest.mark.parametrize("n", LENGTHS)
-def test_schedule_matches_dft(emul_output, n):
-    _, npass, ef, ei, inplace = emul_output[n]
+@pytest.mark.parametrize("n,v", [(n, 0) for n in LENGTHS] + VARIANTS)
+def test_schedule_matches_dft(emul_output, n, v):
+    _, _, npass, ef, ei, inplace, _t = emul_output[(n, v)]
     assert int(npass) <= 3
     # double-precision butterflies: error grows ~ log n * eps * |x|_1
     assert float(ef) < 1e-13 * n and float(ei) < 1e-13 * n
     # the last pass writes each thread's outputs to its own input elements
     # (k_xline writes the transform back without a barrier on this)
     assert inplace == "1"
+
+
+def test_variants_differ_from_the_default_plan(emul_output):
+    for n, v in VARIANTS:
+        assert emul_output[(n, v)][6] != emul_output[(n, 0)][6] or \
+            emul_output[(n, v)][2] != emul_output[(n, 0)][2], (n, v)
