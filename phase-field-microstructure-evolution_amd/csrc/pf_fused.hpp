@@ -22,13 +22,12 @@ struct UPar {
 
 // One simulation of a batched ensemble (a device table of these drives the
 // BATCH forms of the step kernels: one launch per stage covers every member of
-// one grid shape): its carried spectrum S, work bank W, FP64 k = 0 modes and
-// parameters.
+// one grid shape): its carried spectrum S (column-tile layout, below), work
+// bank W, FP64 k = 0 modes and parameters.
 template <typename R> struct Member {
   c2<R>* Wm;
   c2<R>* Wn;
-  c2<R>* Sm;
-  c2<R>* Sn;
+  c2<R>* S;
   double* dc64;
   UPar u;
 };
@@ -461,6 +460,13 @@ k_ypass(c2<R>* __restrict__ main, c2<R>* __restrict__ nyq, long M, long npanels,
 // The spectrum values a thread holds after the forward transform are updated in
 // registers (S read and written at the same rows); when the transform's output
 // rows differ from its input rows an LDS exchange precedes the inverse.
+// S is stored in the kernel's own tile order: tile tt of field f (tt < tm: main
+// block columns tt*TX .., else Nyquist-plane columns) is NZ rows of TX complex,
+// contiguous, at S.main + ((f*(tm+tn) + tt)*NZ + z)*TX + column (Geo::stx, stm,
+// stpf describe it).  Every tile of S is then one contiguous 32 KiB (FP32,
+// NZ = 256) block rather than NZ row segments one z plane apart.  At 256^3 x 17
+// a copy of the pass's data movement alone went 0.99 -> 0.87 ms; the pass
+// itself, with the buffer addressing below, 0.962 -> 0.929 ms.
 // The wavenumbers and the update coefficients are evaluated in FP64 in both
 // precisions: in FP32 a rounded coefficient A = 1/(1 + dt M kappa k^4) would
 // bias every step the same way, and over 1000 steps of a slowly decaying mode
@@ -477,9 +483,11 @@ __device__ __forceinline__ double recip_fast(double x) {
   return r * (2.0 - x * r);
 }
 
-// Minimum waves per SIMD asked of the register allocator: FP32 fits 3 waves
-// (158 VGPRs for the plain update, 168 with the Parseval sums, no spills) with up
-// to 16 complex values per thread; plans holding more (E > 16) and FP64 ask for 2.
+// Minimum waves per SIMD asked of the register allocator: 3 for FP32 plans with
+// up to 16 complex values per thread, 2 for plans holding more (E > 16) and for
+// FP64.  At 256 the FP32 update then takes 116 VGPRs (127 with the Parseval
+// sums) and runs 4 waves per SIMD; asking for 4 instead made the allocator fill
+// all 128 registers and spill (4-8 per lane).
 template <typename R, bool PARSEVAL, int E = 16> constexpr int kZlineWaves = (sizeof(R) == 4 && E <= 16) ? 3 : 2;
 
 // The update coefficients of one parameter set, FP64.
@@ -502,8 +510,13 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
   static_assert(!BATCH || ROLE == ROLE_REG, "batched members step without the mobility chain");
   constexpr int TX = col_tile(NZ, sizeof(c2<R>));
   constexpr int NI = kInSlots<NZ, V>, NO = kOutSlots<NZ, V>, E = plan_E(NZ, V);
-  __shared__ c2<R> sh[(plan_np(NZ, V) > 1) ? NZ * TX : 1];
-  __shared__ double red[16];
+  constexpr unsigned CB = sizeof(c2<R>);
+  static_assert(last_pass_in_place<NZ, V>(), "slot rows of the last pass are t + a per-slot constant");
+  // the exchange tile (up to 64 KiB), which after the tile loop holds the
+  // block reduction's 16 doubles
+  constexpr int SHN = (plan_np(NZ, V) > 1) ? NZ * TX : 1;
+  __shared__ c2<R> sh[SHN * sizeof(c2<R>) >= 16 * sizeof(double) ? SHN : 16 * sizeof(double) / sizeof(c2<R>)];
+  double* const red = reinterpret_cast<double*>(sh);
   const long tpf = tm + tn;
   const long total = (long)nf * tpf;
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
@@ -521,7 +534,7 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
     if constexpr (BATCH) {
       const long m = f / g.F;
       f -= (int)(m * g.F);
-      Sb = Bank<R>{mt[m].Sm, mt[m].Sn};
+      Sb = Bank<R>{mt[m].S, nullptr};
       Wb = Bank<R>{mt[m].Wm, mt[m].Wn};
       dcb = mt[m].dc64;
       zc = zco(mt[m].u);
@@ -532,21 +545,28 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
     const long col0 = (isn ? tt - tm : tt) * TX;
     const int ncol = (int)((Wd - col0) < TX ? (Wd - col0) : TX);
     const bool act = c < ncol;
-    const long off = (isn ? (long)f * g.plane : (long)f * g.hmain) + col0 + c;
-    const long off0 = col0 + c;  // single-field banks X, Y (c-only roles)
-    c2<R>* const wb = (isn ? Wb.nyq : Wb.main) + off;
-    c2<R>* const sb = (isn ? Sb.nyq : Sb.main) + off;
-    c2<R>* const xb = (isn ? X.nyq : X.main) + off0;
-    const c2<R>* const ib = (ROLE == ROLE_CMOB) ? (isn ? Y.nyq : Y.main) + off0 : wb;
+    // Buffer descriptors on the tile's first column (block-uniform): the
+    // natural-layout banks W, X, Y (z rows Wd columns apart; X, Y hold field c
+    // only) and the tile of S.  A thread's row of slot s is its own t plus a
+    // per-slot constant (first pass: in_idx; last pass, in place: out_idx), so
+    // every access is one lane offset plus a uniform one.
+    const unsigned rowb = (unsigned)Wd * CB, nb = (unsigned)NZ * rowb;
+    const BufRes rw = buf_res((isn ? Wb.nyq : Wb.main) + (isn ? (long)f * g.plane : (long)f * g.hmain) + col0, nb);
+    const BufRes rx = buf_res((isn ? X.nyq : X.main) + col0, nb);
+    const BufRes ry = buf_res((isn ? Y.nyq : Y.main) + col0, nb);
+    const BufRes rs = buf_res(Sb.main + ((long)f * tpf + tt) * ((long)NZ * TX), (unsigned)(NZ * TX) * CB);
+    const unsigned lw = ((unsigned)t * (unsigned)Wd + (unsigned)c) * CB;  // natural layout
+    const unsigned ls = (unsigned)threadIdx.x * CB;                        // S tile: row t, column c
     // 1. all loads of the tile at once: transform input, spectrum (and f^) at the output rows
 #pragma unroll
     for (int s = 0; s < NI; ++s)
-      if (act && rfft_in_valid<NZ, V>(t, s)) v[s] = ld_nt<R>(ib + (long)rfft_in_idx<NZ, V>(t, s) * Wd);
+      if (act && rfft_in_valid<NZ, V>(t, s))
+        v[s] = buf_ld<R>((ROLE == ROLE_CMOB) ? ry : rw, lw, (unsigned)rfft_in_idx<NZ, V>(0, s) * rowb);
 #pragma unroll
     for (int s = 0; s < NO; ++s)
       if (act && rfft_out_valid<NZ, V>(t, s)) {
-        sv[s] = ld_nt<R>(sb + (long)rfft_out_idx<NZ, V>(t, s) * Wd);
-        if constexpr (ROLE == ROLE_CMOB) xv[s] = xb[(long)rfft_out_idx<NZ, V>(t, s) * Wd];
+        sv[s] = buf_ld<R>(rs, ls, (unsigned)(rfft_out_idx<NZ, V>(0, s) * TX) * CB);
+        if constexpr (ROLE == ROLE_CMOB) xv[s] = buf_ld<R, 0>(rx, lw, (unsigned)rfft_out_idx<NZ, V>(0, s) * rowb);
       }
     const long col = col0 + c;
     const int kx = isn ? g.M : (int)(col % g.M);
@@ -571,11 +591,10 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
       const int e = rfft_out_idx<NZ, V>(t, s);
       const KCoefR<double> k = kcoef_col<double>(ktz, kc, e);
       const c2<R> w = v[s];
-      const long ga = (long)e * Wd;
       c2<R> sn;
       if constexpr (ROLE == ROLE_GHAT) {
         const c2<R> ch = sv[s];
-        xb[ga] = w;  // f^
+        buf_st<R, 0>(rx, lw, (unsigned)rfft_out_idx<NZ, V>(0, s) * rowb, w);  // f^
         sn = mk<R>((R)(-(k.s1 * w.y) + zc.kc * ch.y * k.k3), (R)(k.s1 * w.x - zc.kc * ch.x * k.k3));
       } else {
         const c2<R> s0 = sv[s];
@@ -611,7 +630,7 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
           if (f != 0) dcb[f] = dcb[f] - zc.Ldt * (double)w.x;
           sn = mk<R>((R)dcb[f], R(0));
         }
-        st_nt<R>(sb + ga, sn);
+        buf_st<R>(rs, ls, (unsigned)(rfft_out_idx<NZ, V>(0, s) * TX) * CB, sn);
 
       }
       v[s] = sn;
@@ -645,10 +664,10 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
         if (act && rfft_in_valid<NZ, V>(t, s)) v[s] = sh[rfft_in_idx<NZ, V>(t, s) * TX + c];
     }
     rfft<R, NZ, true, TX, V>(v, t, act, sh + c, tw);
-    c2<R>* const ob = (ROLE == ROLE_GHAT) ? (isn ? Y.nyq : Y.main) + off0 : wb;
 #pragma unroll
     for (int s = 0; s < NO; ++s)
-      if (act && rfft_out_valid<NZ, V>(t, s)) st_nt<R>(ob + (long)rfft_out_idx<NZ, V>(t, s) * Wd, v[s]);
+      if (act && rfft_out_valid<NZ, V>(t, s))
+        buf_st<R>((ROLE == ROLE_GHAT) ? ry : rw, lw, (unsigned)rfft_out_idx<NZ, V>(0, s) * rowb, v[s]);
     __syncthreads();
   }
   if (PARSEVAL) {

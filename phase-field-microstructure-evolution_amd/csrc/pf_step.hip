@@ -394,8 +394,9 @@ __global__ void k_ac_faithful(Geo g, UCoef u, KTab kt, Bank<double> st, Bank<dou
 // Reduced k-space update for all fields, in place on S, copy into W for the
 // inverse transform; optional Parseval partial sums of the new state for the
 // next step's f_int.  FP32 form keeps each field's k = 0 mode in FP64 (dc64).
+// STILED: S is in the column-tile layout (elem_tiled).
 // ---------------------------------------------------------------------------
-template <typename R, bool DC64, bool MOB, bool PARSEVAL, bool UPD>
+template <typename R, bool DC64, bool MOB, bool PARSEVAL, bool UPD, bool STILED = false>
 __global__ void __launch_bounds__(NB)
 k_update(Geo g, UCoef u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> H, double* __restrict__ dc64,
          double* __restrict__ part, long nrows) {
@@ -406,7 +407,7 @@ k_update(Geo g, UCoef u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> H, double* __res
     const double kap = (r.f == 0) ? u.kc : u.kphi;
     for (int kx = threadIdx.x; kx <= g.M; kx += blockDim.x) {
       const KCoef k = kcoef(kt, kx, r.ky, r.kz);
-      c2<R>* sp = elem(S, g, r.f, kx, r.ky, r.kz);
+      c2<R>* sp = STILED ? elem_tiled(S, g, r.f, kx, r.ky, r.kz) : elem(S, g, r.f, kx, r.ky, r.kz);
       c2<R> sn = *sp;
       if (UPD) {
         const c2<R> w = *elem(W, g, r.f, kx, r.ky, r.kz);
@@ -444,6 +445,29 @@ k_update(Geo g, UCoef u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> H, double* __res
   if (PARSEVAL) {
     const double s = block_sum<NB>(acc, red);
     if (threadIdx.x == 0) part[blockIdx.x] = s * (double)g.P;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A spectrum in the natural layout (main block, Nyquist planes) -> the
+// column-tile layout of S; the tiles' padding columns are zero.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(NB) k_tile_spectrum(Geo g, const c2<R>* __restrict__ main,
+                                                      const c2<R>* __restrict__ nyq, c2<R>* __restrict__ st, long n) {
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n; o += (long)gridDim.x * blockDim.x) {
+    const long c = o % g.stx, rest = o / g.stx;
+    const long kz = rest % g.nz, tf = rest / g.nz;
+    const long f = tf / g.stpf, tt = tf % g.stpf;
+    c2<R> v = mk<R>(R(0), R(0));
+    if (tt < g.stm) {
+      const long col = tt * g.stx + c;
+      if (col < (long)g.M * g.ny) v = main[f * g.hmain + kz * ((long)g.M * g.ny) + col];
+    } else {
+      const long col = (tt - g.stm) * g.stx + c;
+      if (col < g.ny) v = nyq[f * g.plane + kz * g.ny + col];
+    }
+    st[o] = v;
   }
 }
 
@@ -534,7 +558,7 @@ struct pf_handle {
   Bank<double> fs{}, fgrad{}, fdfdc{}, fdum{}, fpsi{};
   double* fmob = nullptr;
   // reduced banks (precision R, stored untyped)
-  void *S_main = nullptr, *S_nyq = nullptr, *W_main = nullptr, *W_nyq = nullptr;
+  void *S_main = nullptr, *W_main = nullptr, *W_nyq = nullptr;  // S: column-tile layout
   void *G_main = nullptr, *G_nyq = nullptr, *mob = nullptr;
   void *Fc_main = nullptr, *Fc_nyq = nullptr, *X_main = nullptr, *X_nyq = nullptr;
   double* dc64 = nullptr;
@@ -953,7 +977,7 @@ int launch_zline(pf_handle* h, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y, doubl
   switch (g.nz) {
 #define PF_CASE(N)                                                                                   \
   case N: {                                                                                          \
-    constexpr int TX = col_tile(N, sizeof(c2<R>));                                                   \
+    constexpr int TX = col_tile(N, sizeof(c2<R>));                                                 \
     const long tm = ((long)g.M * g.ny + TX - 1) / TX, tn = ((long)g.ny + TX - 1) / TX;               \
     const long cap = h->tune.zline_grid > 0 ? h->tune.zline_grid : kZlineMaxGrid;                    \
     long grid = std::min<long>((tm + tn) * nf, cap);                                                 \
@@ -1069,7 +1093,7 @@ int zline_batch(const Geo& g, const UPar& u, const pf_tuning& tun, const KTab& k
   switch (g.nz) {
 #define PF_CASE(N)                                                                                   \
   case N: {                                                                                          \
-    constexpr int TX = col_tile(N, sizeof(c2<float>));                                               \
+    constexpr int TX = col_tile(N, sizeof(c2<float>));                                             \
     const long tm = ((long)g.M * g.ny + TX - 1) / TX, tn = ((long)g.ny + TX - 1) / TX;               \
     const long nf = nmem * g.F;                                                                      \
     long grid = std::min<long>((tm + tn) * nf, cap);                                                 \
@@ -1109,14 +1133,14 @@ int step_reduced(pf_handle* h, bool diag, long rec, bool fint_next) {
   hipStream_t s = h->s;
   const bool mob = h->p.D_gb != 0.0;
   constexpr bool DC = sizeof(R) == 4;
-  Bank<R> S{(c2<R>*)h->S_main, (c2<R>*)h->S_nyq}, W{(c2<R>*)h->W_main, (c2<R>*)h->W_nyq};
+  Bank<R> S{(c2<R>*)h->S_main, nullptr}, W{(c2<R>*)h->W_main, (c2<R>*)h->W_nyq};
   Bank<R> G{(c2<R>*)h->G_main, (c2<R>*)h->G_nyq}, Fc{(c2<R>*)h->Fc_main, (c2<R>*)h->Fc_nyq};
   int rc;
 
   if (diag && !h->fint_ready) {
     tbeg(h, ST_DIAG);
     const long rows = (long)g.plane * g.F;
-    hipLaunchKernelGGL((k_update<R, false, false, true, false>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
+    hipLaunchKernelGGL((k_update<R, false, false, true, false, true>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
                        h->u, h->kt, S, W, G, h->dc64, h->part_fi, rows);
     h->n_part_fi = grid_for(rows, 1);
     tend(h);
@@ -1171,8 +1195,12 @@ int step_reduced(pf_handle* h, bool diag, long rec, bool fint_next) {
 }
 
 template <typename R> int alloc_reduced(pf_handle* h) {
+  Geo& g = h->g;
+  g.stx = col_tile(g.nz, sizeof(c2<R>));
+  g.stm = ((long)g.M * g.ny + g.stx - 1) / g.stx;
+  g.stpf = g.stm + ((long)g.ny + g.stx - 1) / g.stx;
   Bank<R> S{}, W{}, G{};
-  int rc = alloc_bank<R>(h, S, h->g.F);
+  int rc = dalloc(h, (void**)&S.main, sizeof(c2<R>) * (size_t)g.stpf * g.nz * g.stx * g.F);
   if (!rc) rc = alloc_bank<R>(h, W, h->g.F);
   Bank<R> Fc{}, X{};
   if (!rc && h->p.D_gb != 0.0) {
@@ -1181,7 +1209,7 @@ template <typename R> int alloc_reduced(pf_handle* h) {
     if (!rc) rc = dalloc(h, &h->mob, sizeof(R) * (size_t)h->g.P);
   }
   if (!rc) rc = alloc_bank<R>(h, X, h->g.F);
-  h->S_main = S.main; h->S_nyq = S.nyq; h->W_main = W.main; h->W_nyq = W.nyq;
+  h->S_main = S.main; h->W_main = W.main; h->W_nyq = W.nyq;
   h->G_main = G.main; h->G_nyq = G.nyq; h->Fc_main = Fc.main; h->Fc_nyq = Fc.nyq;
   h->X_main = X.main; h->X_nyq = X.nyq;
   if (!rc) rc = dalloc(h, (void**)&h->dc64, sizeof(double) * h->g.F);
@@ -1230,12 +1258,16 @@ std::vector<double> host_means(const double* phi, const double* con, long P, int
 }
 
 // Reduced forms, once W holds the real state (SoA): S <- F(W) with f_trans
-// normalisation, W <- x-half lines, dc64 <- the FP64 field means dc.
+// normalisation (computed in X, then put in the column-tile layout), W <- x-half
+// lines, dc64 <- the FP64 field means dc.
 template <typename R> int finish_put_reduced(pf_handle* h, const std::vector<double>& dc) {
   const Geo& g = h->g;
-  PF_HIP(hipMemcpyAsync(h->S_main, h->W_main, sizeof(R) * (size_t)g.P * g.F, hipMemcpyDeviceToDevice, h->s));
-  int rc = fwd3d<R>((c2<R>*)h->S_main, (c2<R>*)h->S_nyq, g.nx, g.ny, g.nz, g.F, (R)(1.0 / (double)(float)g.P), h->s);
+  PF_HIP(hipMemcpyAsync(h->X_main, h->W_main, sizeof(R) * (size_t)g.P * g.F, hipMemcpyDeviceToDevice, h->s));
+  int rc = fwd3d<R>((c2<R>*)h->X_main, (c2<R>*)h->X_nyq, g.nx, g.ny, g.nz, g.F, (R)(1.0 / (double)(float)g.P), h->s);
   if (rc) return rc;
+  const long nst = g.stpf * g.nz * g.stx * g.F;
+  hipLaunchKernelGGL((k_tile_spectrum<R>), dim3(grid_for(nst, NB)), dim3(NB), 0, h->s, g,
+                     (const c2<R>*)h->X_main, (const c2<R>*)h->X_nyq, (c2<R>*)h->S_main, nst);
   // W <- x-half lines: (1/nx) R2C along x of the real state == z,y-inverse of S
   rc = x_r2c<R>(g.nx, (c2<R>*)h->W_main, (c2<R>*)h->W_nyq, (long)g.ny * g.nz * g.F, (R)(1.0 / (double)g.nx), h->s);
   if (rc) return rc;
@@ -1646,7 +1678,7 @@ int upload_table(Group& gr) {
   for (size_t i = 0; i < gr.mem.size(); ++i) {
     pf_handle* h = gr.mem[i];
     tab[i] = Member<float>{(c2<float>*)h->W_main, (c2<float>*)h->W_nyq, (c2<float>*)h->S_main,
-                           (c2<float>*)h->S_nyq, h->dc64, h->u};
+                           h->dc64, h->u};
   }
   if (hipMalloc(&gr.table, sizeof(Member<float>) * tab.size()) != hipSuccess) {
     gr.table = nullptr;
@@ -1841,16 +1873,16 @@ extern "C" int pf_diagnose(pf_handle* h, pf_diag* out) {
   } else {
     const bool f32 = h->mode == PF_MODE_FP32_MIXED;
     if (f32) {
-      Bank<float> S{(c2<float>*)h->S_main, (c2<float>*)h->S_nyq};
-      hipLaunchKernelGGL((k_update<float, false, false, true, false>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
+      Bank<float> S{(c2<float>*)h->S_main, nullptr};
+      hipLaunchKernelGGL((k_update<float, false, false, true, false, true>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
                          h->u, h->kt, S, S, S, h->dc64, h->part_fi, rows);
       const size_t shb = sizeof(float) * (size_t)g.var * NB + 16 + sizeof(double) * NB;
       if ((rc = real_state_reduced<float>(h))) return rc;
       hipLaunchKernelGGL((k_nl_reduced<float, true, false, false>), dim3(gr_nl), dim3(NB), shb, s, g, h->u,
                          (float*)h->X_main, (float*)nullptr, h->part_nl);
     } else {
-      Bank<double> S{(c2<double>*)h->S_main, (c2<double>*)h->S_nyq};
-      hipLaunchKernelGGL((k_update<double, false, false, true, false>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
+      Bank<double> S{(c2<double>*)h->S_main, nullptr};
+      hipLaunchKernelGGL((k_update<double, false, false, true, false, true>), dim3(grid_for(rows, 1)), dim3(NB), 0, s, g,
                          h->u, h->kt, S, S, S, h->dc64, h->part_fi, rows);
       const size_t shb = sizeof(double) * (size_t)g.var * NB + 16 + sizeof(double) * NB;
       if ((rc = real_state_reduced<double>(h))) return rc;

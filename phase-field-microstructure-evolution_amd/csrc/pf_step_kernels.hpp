@@ -84,6 +84,11 @@ template <typename R> struct Bank {
 struct Geo {
   int nx, ny, nz, M, var, F;
   long P, plane, hmain;  // nx*ny*nz, ny*nz, M*ny*nz
+  // column-tile layout of the reduced forms' carried spectrum S: tiles of stx
+  // columns (the z-line kernel's tile width), stm of them over the main block's
+  // M*ny columns, stpf per field (stm + those of the ny Nyquist-plane columns)
+  int stx;
+  long stm, stpf;
 };
 
 // Row decode for k-space kernels: one block per (ky, kz, field) row, threads over
@@ -103,6 +108,13 @@ template <typename R>
 __device__ __forceinline__ c2<R>* elem(const Bank<R>& b, const Geo& g, int f, int kx, int ky, int kz) {
   if (kx < g.M) return b.main + (long)f * g.hmain + ((long)ky + (long)g.ny * kz) * g.M + kx;
   return b.nyq + (long)f * g.plane + ky + (long)g.ny * kz;
+}
+// The same element of a spectrum in the column-tile layout (all of it in b.main).
+template <typename R>
+__device__ __forceinline__ c2<R>* elem_tiled(const Bank<R>& b, const Geo& g, int f, int kx, int ky, int kz) {
+  const long col = (kx < g.M) ? (long)ky * g.M + kx : (long)ky;
+  const long tt = (kx < g.M ? 0 : g.stm) + col / g.stx;
+  return b.main + (((long)f * g.stpf + tt) * g.nz + kz) * g.stx + col % g.stx;
 }
 
 // Deterministic block reduction helpers (fixed tree order).
