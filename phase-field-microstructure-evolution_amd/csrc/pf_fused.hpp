@@ -485,9 +485,8 @@ __device__ __forceinline__ double recip_fast(double x) {
 
 // Minimum waves per SIMD asked of the register allocator: 3 for FP32 plans with
 // up to 16 complex values per thread, 2 for plans holding more (E > 16) and for
-// FP64.  At 256 the FP32 update then takes 116 VGPRs (127 with the Parseval
-// sums) and runs 4 waves per SIMD; asking for 4 instead made the allocator fill
-// all 128 registers and spill (4-8 per lane).
+// FP64.  At 256 the FP32 update takes 165 VGPRs (3 waves) with no spills;
+// capped at 4 waves it spilled 6-25 registers per lane and ran 16% slower.
 template <typename R, bool PARSEVAL, int E = 16> constexpr int kZlineWaves = (sizeof(R) == 4 && E <= 16) ? 3 : 2;
 
 // The update coefficients of one parameter set, FP64.
@@ -572,24 +571,19 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
     const int kx = isn ? g.M : (int)(col % g.M);
     const int ky = isn ? (int)col : (int)(col / g.M);
     const KCol<double> kc = kcol<double>(kt, kx, ky);
-    // The z-table entries a thread needs are the same for every tile; an opaque
-    // zero offset keeps the compiler from hoisting all of them (2 doubles per
-    // slot) into registers for the whole loop, re-reading them from L1 instead.
-    KTab ktz = kt;
-    {
-      long zo = 0;
-      asm volatile("" : "+s"(zo));
-      ktz.az += zo;
-      ktz.azp += zo;
-    }
     const double kap = (f == 0) ? zc.kc : zc.kphi;
+    // The z wavenumbers of a thread's rows are the same for every tile; an
+    // opaque copy of t keeps the compiler from hoisting all of them (2 doubles
+    // per slot) out of the tile loop into registers, recomputing them instead.
+    int tz = t;
+    asm volatile("" : "+v"(tz));
     // 2. forward z, update in registers
     rfft<R, NZ, false, TX, V>(v, t, act, sh + c, tw);
 #pragma unroll
     for (int s = 0; s < NO; ++s) {
       if (!(act && rfft_out_valid<NZ, V>(t, s))) continue;
       const int e = rfft_out_idx<NZ, V>(t, s);
-      const KCoefR<double> k = kcoef_col<double>(ktz, kc, e);
+      const KCoefR<double> k = kcoef_z(kc, rfft_out_idx<NZ, V>(tz, s), NZ, kt.uz);
       const c2<R> w = v[s];
       c2<R> sn;
       if constexpr (ROLE == ROLE_GHAT) {
@@ -645,7 +639,8 @@ k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
         if (!(act && rfft_out_valid<NZ, V>(t, s))) continue;
         const R m2 = v[s].x * v[s].x + v[s].y * v[s].y;
         const int e = rfft_out_idx<NZ, V>(t, s);
-        const R zp = (2 * e == NZ) ? R(0) : (R)ktz.az[e];  // a'_z: a_z zeroed at Nyquist
+        const int ez = rfft_out_idx<NZ, V>(tz, s);
+        const R zp = (2 * ez == NZ) ? R(0) : (R)az_of(ez, NZ, kt.uz);  // a'_z: a_z zeroed at Nyquist
         pa1 += m2;
         pa2 += zp * zp * m2;
       }

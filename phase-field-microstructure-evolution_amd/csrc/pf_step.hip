@@ -1399,6 +1399,7 @@ extern "C" int pf_create(const pf_params* p, int nx, int ny, int nz, int var, in
     h->kt.ax = d; h->kt.axp = d + M + 1; d += 2 * (M + 1);
     h->kt.ay = d; h->kt.ayp = d + ny; d += 2 * ny;
     h->kt.az = d; h->kt.azp = d + nz;
+    h->kt.uz = (2.0 * (std::atan(1.0) * 4.0)) / (double)(float)nz;  // the unit of kaxis
     std::vector<float> tabf(tab.begin(), tab.end());
     if ((rc = dalloc(h, (void**)&h->dkf, sizeof(float) * tabf.size()))) return bail(rc);
     if (hipMemcpy(h->dkf, tabf.data(), sizeof(float) * tabf.size(), hipMemcpyHostToDevice) != hipSuccess)

@@ -8,6 +8,7 @@ namespace pf {
 // for kx in [0, nx/2], ky in [0, ny), kz in [0, nz)   (src/k_space.f90:21-59).
 struct KTab {
   const double *ax, *axp, *ay, *ayp, *az, *azp;
+  double uz;  // 2 pi / nz: az[m] = (double)(float)m_s * uz, m_s = m or m - nz
 };
 // The same tables rounded to FP32, for the FP32-mixed update arithmetic.
 struct KTabF {
@@ -45,6 +46,29 @@ __device__ __forceinline__ KCoefR<R> kcoef_col(const KT& t, const KCol<R>& c, in
   const R az = t.az[kz], azp = t.azp[kz];
   const R z2 = az * az;
   KCoefR<R> k;
+  k.s1 = c.s1 + azp;
+  k.k2 = c.k2 + -z2;
+  k.k3 = c.k3 + -(azp * z2);
+  k.k4 = c.k4 + (z2 * az) * az;
+  k.kp2 = c.kp2 + azp * azp;
+  return k;
+}
+
+// kcoef_col with a_z and a'_z evaluated instead of read from the table.  The
+// table entry is the one product (double)(float)m_s * (2 pi / nz) (kaxis); the
+// product here is kept out of FMA contraction, so it is rounded on its own and
+// equals the table value bit for bit.  The z-line kernel so avoids a load per
+// slot whose wait would also wait for the stores issued before it.  (What the
+// compiler then contracts downstream, e.g. 1 + D dt kappa k^4 into one FMA or
+// not, depends on the surrounding code: the FP64 state moved by ~1e-16.)
+__device__ __forceinline__ double az_of(int kz, int nz, double uz) {
+#pragma clang fp contract(off)
+  return (double)(float)((2 * kz <= nz) ? kz : kz - nz) * uz;
+}
+__device__ __forceinline__ KCoefR<double> kcoef_z(const KCol<double>& c, int kz, int nz, double uz) {
+  const double az = az_of(kz, nz, uz), azp = (2 * kz == nz) ? 0.0 : az;
+  const double z2 = az * az;
+  KCoefR<double> k;
   k.s1 = c.s1 + azp;
   k.k2 = c.k2 + -z2;
   k.k3 = c.k3 + -(azp * z2);
