@@ -190,18 +190,18 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
 // a spare: the threads of the last wave beyond the njobs*T that carry the
 // transforms run the same schedule on it, so no transform slot is predicated.
 // Staging and the final store move whole rows, one per wave at a time.
-// Dynamic LDS: ((njobs+1)*MP + M) complex + (njobs+1) reals + 16 doubles.
+// The table W_NX^k, k < NX, is copied to LDS once per block: the real<->half-
+// complex pairing reads k < M, the x transforms (length M, stride 2) the rest.
+// Dynamic LDS: ((njobs+1)*MP + NX) complex + (njobs+1) reals + 16 doubles.
 // ---------------------------------------------------------------------------
-// Threads per x-line block: FP32 blocks may use 1024 threads (the kernel fits in
-// 128 VGPRs), which a 512^3 grid with 37 fields needs (37 transforms x 16
-// threads); FP64 blocks stop at 512.
+// Threads per x-line block: FP32 blocks may use 1024 threads, which a 512^3 grid
+// with 37 fields needs (37 transforms x 16 threads); FP64 blocks stop at 512.
 template <typename R> constexpr int kXlineMaxThreads = sizeof(R) == 4 ? 1024 : 512;
-// Minimum waves per SIMD asked of the register allocator.  FP32: the 1024-thread
-// bound already means 4 (128 VGPRs).  FP64: uncapped the kernel takes 202 VGPRs
-// (2 waves/SIMD, two 3-wave blocks per CU); capped at 3 waves it spills 120
-// bytes per lane and still runs faster, 1.52 -> 1.32 ms at 256^3 x 17 (4 waves
-// spill 292 bytes: 1.91 ms).
-template <typename R> constexpr int kXlineWaves = sizeof(R) == 4 ? 4 : 3;
+// Minimum waves per SIMD asked of the register allocator.  FP32: 5 (at most 102
+// VGPRs; the 256-point kernel takes 92, so 5 three-wave blocks fit a CU's
+// SIMDs with room for a sixth).  FP64: 3 (the 256-point kernel takes 141 VGPRs
+// with no spills).
+template <typename R> constexpr int kXlineWaves = sizeof(R) == 4 ? 5 : 3;
 // Rows each wave keeps in flight when staging a line group.
 constexpr int kXlineRows = 2;
 
@@ -227,22 +227,13 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const int F = XONLY ? 1 : g.F;
   const int njobs = F * LINES;
   c2<R>* sh = reinterpret_cast<c2<R>*>(smem);
-  c2<R>* shw = sh + (long)(njobs + 1) * MP;  // W^k, k < M (row njobs: idle threads' row)
-  R* shn = reinterpret_cast<R*>(shw + M);
-  double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)(njobs + 1) * MP + M) +
+  c2<R>* shw = sh + (long)(njobs + 1) * MP;  // W_NX^k, k < NX (row njobs: idle threads' row)
+  R* shn = reinterpret_cast<R*>(shw + NX);
+  double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)(njobs + 1) * MP + NX) +
                                                     sizeof(R) * (njobs + 1)) + 15) & ~(size_t)15));
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  const int nt = blockDim.x, nw = nt >> 6;
   constexpr int KPL = (M + 63) / 64;  // row elements per lane when a wave moves a row
-  const int job = tid % njobs, tq = tid / njobs;
-  for (int k = tid; k < M; k += nt) shw[k] = tw[k];
-  // threads beyond the T per job run the same schedule on a spare row, so the
-  // transform phases carry no per-slot predicates
-  const bool act = tq < T;
-  const int t = act ? tq : tq % T;
-  __builtin_assume(t >= 0 && t < T);
-  const int row = act ? job : njobs;
-  c2<R>* col = sh + row * MP;
+  for (int k = threadIdx.x; k < NX; k += nt) shw[k] = tw[k];
   const R scale = (R)u.invP;
   double a_ch = 0.0, a_eta = 0.0;
   R cmax = 0, cmin = 1, pmin = 1, p3 = 0;
@@ -252,6 +243,21 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const long chunk = (ngroups + gridDim.x - 1) / gridDim.x;
   const long lg_end = ((lb + 1) * chunk < ngroups) ? (lb + 1) * chunk : ngroups;
   for (long lg = lb * chunk; lg < lg_end; ++lg) {
+    // Per-thread indices are recomputed each iteration from an opaque copy of
+    // the thread id: hoisted, the addresses derived from them stayed live across
+    // the loop (and the transforms' twiddles, read from global memory, too),
+    // 126 VGPRs at 256^3 x 17; recomputed, with the twiddles from LDS, 90.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, wv = tid >> 6;
+    const int job = tid % njobs, tq = tid / njobs;
+    // threads beyond the T per job run the same schedule on a spare row, so the
+    // transform phases carry no per-slot predicates
+    const bool act = tq < T;
+    const int t = act ? tq : tq % T;
+    __builtin_assume(t >= 0 && t < T);
+    const int row = act ? job : njobs;
+    c2<R>* col = sh + row * MP;
     long L0 = lg * LINES, m = 0;
     c2<R>* wmm = wm;
     c2<R>* wnn = wn;
@@ -308,7 +314,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     //    thread's outputs at exactly the elements it loaded as that pass's inputs
     //    (out_idx = in_idx when j < NB = LS), so no thread overwrites an element
     //    another thread has yet to read.
-    rfft<R, M, true, 1>(v, t, true, col, tw, 2);
+    rfft<R, M, true, 1>(v, t, true, col, shw, 2);
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
       if (rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
@@ -355,7 +361,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
 #pragma unroll
     for (int s = 0; s < kInSlots<M>; ++s)
       if (rfft_in_valid<M>(t, s)) v[s] = col[rfft_in_idx<M>(t, s)];
-    rfft<R, M, false, 1>(v, t, true, col, tw, 2);  // write-back without a barrier, as in 3.
+    rfft<R, M, false, 1>(v, t, true, col, shw, 2);  // write-back without a barrier, as in 3.
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
       if (rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
@@ -379,6 +385,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     }
     __syncthreads();
   }
+  const int tid = threadIdx.x;
   if (!XONLY && diag) {
     const double s0 = bsum(a_ch, red), s1 = bsum(a_eta, red);
     const double e0 = bext<true>((double)cmax, red), e1 = bext<false>((double)cmin, red);
