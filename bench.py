@@ -117,14 +117,15 @@ def roofline_of(stages, steps):
     return name, avg_ms, achieved, st["bytes"]
 
 
-def stage_kernels(n, lines=1):
+def stage_kernels(n, lines=1, plan=0):
     """Kernel behind each timed stage of the FP32 step on an n^3 grid, named as
-    rocprofv3 prints them (lines: x lines per x-line block)."""
+    rocprofv3 prints them (lines: x lines per x-line block; plan: the z
+    transforms' radix plan variant)."""
     return {
         "nonlinear": [f"pf::k_xline<float, {n}, {lines}, false, false>"],
         "y_fwd": [f"pf::k_ypass<float, {n}, false, false>"],
-        "kspace_update": [f"pf::k_zline<float, {n}, 0, true, false, false>",
-                          f"pf::k_zline<float, {n}, 0, true, true, false>"],
+        "kspace_update": [f"pf::k_zline<float, {n}, 0, true, false, false, {plan}>",
+                          f"pf::k_zline<float, {n}, 0, true, true, false, {plan}>"],
         "y_inv": [f"pf::k_ypass<float, {n}, true, false>"],
     }
 
@@ -448,7 +449,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name, n, var),
                      "kernel_source_sha16": kernel_source_sha16(),
-                     "stage": name, "kernel": " / ".join(stage_kernels(n, lines).get(name, [name])),
+                     "stage": name, "kernel": " / ".join(stage_kernels(n, lines, tuning["zline_plan"]).get(name, [name])),
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": kbytes,
                      "stream_reference": {"achieved": stream, "frac": achieved / stream if stream else None,
                                           "kernel": "torch in-place multiply over the same bytes "
