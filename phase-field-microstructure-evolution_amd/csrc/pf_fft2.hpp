@@ -372,7 +372,37 @@ __host__ __device__ __forceinline__ void rpass_compute(c2<R>* v, int t, bool act
 // LDS element stride: compile-time ES, or ES == 0 for a runtime stride es.
 template <int ES> __device__ __forceinline__ int lds_stride(int es) { return ES ? ES : es; }
 
-template <typename R, int N, int P, int ES, int V = 0>
+// Exchange flags XF of a transform whose LDS line is private to one wave:
+//   XF_WAVE: the T threads of each line are lanes of one wave, so an exchange
+//            orders the wave's own LDS accesses (wave-scope fences) instead of
+//            synchronising the block;
+//   XF_SWZ:  element e of the line is stored at lds_swz(e) = e + e / 16 (one pad
+//            element after every 16), which spreads the exchange accesses of
+//            neighbouring threads, 16 elements apart in a radix-16 pass, over the
+//            LDS banks; a line of N elements spans N + N / 16.  Added rather than
+//            XORed, the pad keeps each access a per-thread base plus an immediate
+//            (an XOR swizzle needed an address register per slot and made the
+//            x-line kernel spill).
+enum { XF_WAVE = 1, XF_SWZ = 2 };
+template <int XF> __host__ __device__ constexpr int lds_swz(int e) {
+  return (XF & XF_SWZ) ? e + (e >> 4) : e;
+}
+template <int XF> __device__ __forceinline__ void xsync() {
+  if constexpr (XF & XF_WAVE) {
+    // the scheduling barrier keeps the passes' code apart as a block barrier
+    // would: interleaved across the exchange, the x-line kernel's passes needed
+    // more than 128 VGPRs
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_sched_barrier(0);
+  } else {
+    __syncthreads();
+  }
+}
+
+template <typename R, int N, int P, int ES, int V = 0, int XF = 0>
 __device__ __forceinline__ void rpass_store(const c2<R>* v, int t, bool act, c2<R>* lds, int es) {
   using C = PassC<N, P, V>;
   const int st = lds_stride<ES>(es);
@@ -380,10 +410,10 @@ __device__ __forceinline__ void rpass_store(const c2<R>* v, int t, bool act, c2<
   for (int q = 0; q < C::Q; ++q) {
     if (!act || !C::valid(t, q)) continue;
 #pragma unroll
-    for (int m = 0; m < C::r; ++m) lds[C::out_idx(t, q, m) * st] = v[q * C::r + m];
+    for (int m = 0; m < C::r; ++m) lds[lds_swz<XF>(C::out_idx(t, q, m)) * st] = v[q * C::r + m];
   }
 }
-template <typename R, int N, int P, int ES, int V = 0>
+template <typename R, int N, int P, int ES, int V = 0, int XF = 0>
 __device__ __forceinline__ void rpass_load(c2<R>* v, int t, bool act, const c2<R>* lds, int es) {
   using C = PassC<N, P, V>;
   const int st = lds_stride<ES>(es);
@@ -391,28 +421,29 @@ __device__ __forceinline__ void rpass_load(c2<R>* v, int t, bool act, const c2<R
   for (int q = 0; q < C::Q; ++q) {
     if (!act || !C::valid(t, q)) continue;
 #pragma unroll
-    for (int m = 0; m < C::r; ++m) v[q * C::r + m] = lds[C::in_idx(t, q, m) * st];
+    for (int m = 0; m < C::r; ++m) v[q * C::r + m] = lds[lds_swz<XF>(C::in_idx(t, q, m)) * st];
   }
 }
 
-// Passes P.. of the transform; each exchange is bracketed by block barriers.
-template <typename R, int N, bool INV, int ES, int P, int V = 0>
+// Passes P.. of the transform; each exchange is bracketed by block barriers, or
+// by wave-scope fences when the line belongs to one wave (XF_WAVE).
+template <typename R, int N, bool INV, int ES, int P, int V = 0, int XF = 0>
 __device__ __forceinline__ void rfft_from(c2<R>* v, int t, bool act, c2<R>* lds, const c2<R>* __restrict__ tw,
                                           int twm, int es) {
   rpass_compute<R, N, INV, P, V>(v, t, act, tw, twm);
   if constexpr (P + 1 < plan_np(N, V)) {
-    __syncthreads();
-    rpass_store<R, N, P, ES, V>(v, t, act, lds, es);
-    __syncthreads();
-    rpass_load<R, N, P + 1, ES, V>(v, t, act, lds, es);
-    rfft_from<R, N, INV, ES, P + 1, V>(v, t, act, lds, tw, twm, es);
+    xsync<XF>();
+    rpass_store<R, N, P, ES, V, XF>(v, t, act, lds, es);
+    xsync<XF>();
+    rpass_load<R, N, P + 1, ES, V, XF>(v, t, act, lds, es);
+    rfft_from<R, N, INV, ES, P + 1, V, XF>(v, t, act, lds, tw, twm, es);
   }
 }
 // ES = 0: the LDS element stride is the runtime value es.  V: plan variant.
-template <typename R, int N, bool INV, int ES, int V = 0>
+template <typename R, int N, bool INV, int ES, int V = 0, int XF = 0>
 __device__ __forceinline__ void rfft(c2<R>* v, int t, bool act, c2<R>* lds, const c2<R>* __restrict__ tw,
                                      int twm = 1, int es = 0) {
-  rfft_from<R, N, INV, ES, 0, V>(v, t, act, lds, tw, twm, es);
+  rfft_from<R, N, INV, ES, 0, V, XF>(v, t, act, lds, tw, twm, es);
 }
 
 // Element index of register slot s after the transform (last pass outputs) and

@@ -183,11 +183,19 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
 //   mobd   : (m - D_mean) per point when mob (input of the mobility chain).
 //   XONLY  : mobility-chain variant for one field: C2R, multiply by mobd, R2C.
 // One job = one (field, line) x transform, done by T = plan_T(M) threads.  The
-// LDS tile is job-major, job j's M complex elements at sh[j*MP + k] with the odd
-// pitch MP = M + 1: lanes of a wave run consecutive jobs (transform phases) or
-// consecutive points (pointwise), so both walk distinct banks, and every address
-// a thread forms is its own row base plus a compile-time offset.  Row njobs is
-// a spare: the threads of the last wave beyond the njobs*T that carry the
+// LDS tile is job-major: job j's M complex elements in row j, pitch MP.
+//   * When T divides 64 and the plan has at most two passes (every length but
+//     M = 80 and 96), a job's T threads are
+//     consecutive lanes of one wave (t = tid % T), so the transforms' exchanges
+//     stay inside the wave and need no block barrier (XF_WAVE).  Element k of a
+//     row sits at lds_swz(k) = k + k / 16 (XF_SWZ) and the pitch MP is chosen
+//     from a model of the LDS banks over every access pattern of the kernel
+//     (exchanges, pairing k with M - k, rows by lanes): the strided exchange
+//     accesses of the T lanes and of the 64 / T jobs of a wave then mostly fall
+//     on distinct banks.
+//   * Otherwise lanes run consecutive jobs (job = tid % njobs) with the odd
+//     pitch MP = M + 1, and each exchange is bracketed by block barriers.
+// Row njobs is a spare: the threads beyond the njobs*T that carry the
 // transforms run the same schedule on it, so no transform slot is predicated.
 // Staging and the final store move whole rows, one per wave at a time.
 // The table W_NX^k, k < NX, is copied to LDS once per block: the real<->half-
@@ -205,7 +213,28 @@ template <typename R> constexpr int kXlineWaves = sizeof(R) == 4 ? 5 : 3;
 // Rows each wave keeps in flight when staging a line group.
 constexpr int kXlineRows = 2;
 
-__host__ __device__ constexpr int xline_row_pitch(int M) { return M | 1; }
+// Wave-local rows for one- and two-pass plans whose T divides 64; the three-pass
+// plan of 96 points spilled 15 VGPRs per lane that way and ran 19% slower at
+// 192^3, so it keeps block barriers, as does 80 (T = 10).
+__host__ __device__ constexpr bool xline_wave_local(int M) { return 64 % plan_T(M) == 0 && plan_np(M) <= 2; }
+__host__ __device__ constexpr int xline_xf(int M) { return xline_wave_local(M) ? (XF_WAVE | XF_SWZ) : 0; }
+// Row pitch in elements of cbytes bytes: the bank-model choices for FP32
+// elements; FP64 rows take the smallest padded pitch (at 512 points they must
+// fit 33 rows of 16-byte elements in LDS).
+__host__ __device__ constexpr int xline_row_pitch(int M, int cbytes) {
+  if (!xline_wave_local(M)) return M | 1;
+  if (cbytes != 8) return M + M / 16 + 1;
+  switch (M) {  // bank-model choices, >= M + M / 16
+    case 24: return 26;
+    case 32: return 38;
+    case 48: return 60;
+    case 64: return 72;
+    case 96: return 104;
+    case 128: return 152;
+    case 256: return 304;
+    default: return M + M / 16 + 1;
+  }
+}
 
 // BATCH: ngroups counts the line groups of every member in mt (member m owns
 // groups m*gpm .. m*gpm + gpm - 1, gpm = plane / LINES), each taking its banks
@@ -221,7 +250,8 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
         const c2<R>* __restrict__ tw, double* __restrict__ part, long ngroups, int diag, int mob,
         const Member<R>* __restrict__ mt) {
   constexpr int M = NX / 2, T = plan_T(M), E = plan_E(M);
-  constexpr int MP = xline_row_pitch(M);
+  constexpr int MP = xline_row_pitch(M, sizeof(c2<R>)), XF = xline_xf(M);
+  constexpr bool WX = xline_wave_local(M);
   static_assert(last_pass_in_place<M>(), "the write-backs after the x transforms rely on it");
   extern __shared__ unsigned char smem[];
   const int F = XONLY ? 1 : g.F;
@@ -250,10 +280,10 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const int lane = tid & 63, wv = tid >> 6;
-    const int job = tid % njobs, tq = tid / njobs;
-    // threads beyond the T per job run the same schedule on a spare row, so the
+    const int job = WX ? tid / T : tid % njobs, tq = WX ? tid % T : tid / njobs;
+    // threads beyond the njobs*T run the same schedule on a spare row, so the
     // transform phases carry no per-slot predicates
-    const bool act = tq < T;
+    const bool act = WX ? job < njobs : tq < T;
     const int t = act ? tq : tq % T;
     __builtin_assume(t >= 0 && t < T);
     const int row = act ? job : njobs;
@@ -290,7 +320,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
         if (jb < njobs) {
 #pragma unroll
           for (int kk = 0; kk < KPL; ++kk)
-            if (M % 64 == 0 || lane + 64 * kk < M) sh[jb * MP + lane + 64 * kk] = tmp[r][kk];
+            if (M % 64 == 0 || lane + 64 * kk < M) sh[jb * MP + lds_swz<XF>(lane + 64 * kk)] = tmp[r][kk];
         }
       }
     }
@@ -304,8 +334,8 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     for (int s = 0; s < kInSlots<M>; ++s) {
       if (!rfft_in_valid<M>(t, s)) continue;
       const int k = rfft_in_idx<M>(t, s);
-      // k = 0 pairs with the Nyquist entry; col[M] is the row's pad element
-      const c2<R> pk = c2r_pre<R>(col[k], col[M - k], shw[k]);
+      // k = 0 pairs with the Nyquist entry (its c2r_pre value is not used)
+      const c2<R> pk = c2r_pre<R>(col[lds_swz<XF>(k)], col[lds_swz<XF>((M - k) % M)], shw[k]);
       const R a0 = col[0].x, xn = shn[row];
       v[s] = (k == 0) ? mk<R>(a0 + xn, a0 - xn) : pk;
     }
@@ -314,10 +344,10 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     //    thread's outputs at exactly the elements it loaded as that pass's inputs
     //    (out_idx = in_idx when j < NB = LS), so no thread overwrites an element
     //    another thread has yet to read.
-    rfft<R, M, true, 1>(v, t, true, col, shw, 2);
+    rfft<R, M, true, 1, 0, XF>(v, t, true, col, shw, 2);
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
-      if (rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
+      if (rfft_out_valid<M>(t, s)) col[lds_swz<XF>(rfft_out_idx<M>(t, s))] = v[s];
     __syncthreads();
     // 4. pointwise: point x of line l is component x%2 of element x/2 of row
     //    f*LINES + l.  When the group has more points than the block has threads
@@ -329,7 +359,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       constexpr long fs = (long)LINES * MP;  // field stride in complex elements
       for (int p = tid; p < LINES * M; p += nt) {
         const int l = p / M, k = p - l * M;
-        c2<R>* vv = sh + (long)l * MP + k;
+        c2<R>* vv = sh + (long)l * MP + lds_swz<XF>(k);
         const long gp = (L0 + l) * NX + 2 * k;  // point index within a field
         R mo[2];
         if (diag) nl_points<R, 2, true>(vv, fs, g.var, uu, mo, a_ch, a_eta, cmax, cmin, pmin, p3);
@@ -344,7 +374,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       constexpr long fs = 2L * LINES * MP;  // field stride in reals
       for (int p = tid; p < LINES * NX; p += nt) {
         const int l = p / NX, x = p - l * NX;
-        R* vv = rl + 2L * l * MP + x;
+        R* vv = rl + 2L * ((long)l * MP + lds_swz<XF>(x >> 1)) + (x & 1);
         const long gp = (L0 + l) * NX + x;
         if (XONLY) {
           vv[0] = vv[0] * mobd[gp];
@@ -360,11 +390,11 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     // 5. forward x transform
 #pragma unroll
     for (int s = 0; s < kInSlots<M>; ++s)
-      if (rfft_in_valid<M>(t, s)) v[s] = col[rfft_in_idx<M>(t, s)];
-    rfft<R, M, false, 1>(v, t, true, col, shw, 2);  // write-back without a barrier, as in 3.
+      if (rfft_in_valid<M>(t, s)) v[s] = col[lds_swz<XF>(rfft_in_idx<M>(t, s))];
+    rfft<R, M, false, 1, 0, XF>(v, t, true, col, shw, 2);  // write-back without a barrier, as in 3.
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
-      if (rfft_out_valid<M>(t, s)) col[rfft_out_idx<M>(t, s)] = v[s];
+      if (rfft_out_valid<M>(t, s)) col[lds_swz<XF>(rfft_out_idx<M>(t, s))] = v[s];
     __syncthreads();
     // 6. R2C post-processing and store (coalesced along k), one row per wave
     const R hs = R(0.5) * scale;
@@ -375,7 +405,8 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
 #pragma unroll
       for (int kk = 0; kk < KPL; ++kk) {
         const int k = lane + 64 * kk;
-        if (M % 64 == 0 || k < M) st_nt<R>(dst + k, r2c_post<R>(rw[k], rw[(M - k) % M], shw[k], hs));
+        if (M % 64 == 0 || k < M)
+          st_nt<R>(dst + k, r2c_post<R>(rw[lds_swz<XF>(k)], rw[lds_swz<XF>((M - k) % M)], shw[k], hs));
       }
     }
     for (int i = tid; i < njobs; i += nt) {

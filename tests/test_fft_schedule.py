@@ -43,6 +43,16 @@ def test_schedule_matches_dft(emul_output, n, v):
     assert inplace == "1"
 
 
+@pytest.mark.parametrize("n", [4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 256])
+def test_swizzled_exchange_matches_dft(emul_output, n):
+    """The x-line rows store element e at lds_swz(e) = e + e/16; the transform
+    with its exchanges through that layout still computes the DFT, and lds_swz
+    maps [0, n) one-to-one into the n + n/16 elements of a row."""
+    _, _, npass, ef, ei, inplace, perm = emul_output[(n, 9)]
+    assert float(ef) < 1e-13 * n and float(ei) < 1e-13 * n
+    assert perm == "1"
+
+
 def test_variants_differ_from_the_default_plan(emul_output):
     for n, v in VARIANTS:
         assert emul_output[(n, v)][6] != emul_output[(n, 0)][6] or \
