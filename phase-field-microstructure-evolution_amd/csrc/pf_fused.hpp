@@ -195,12 +195,15 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
 //     on distinct banks.
 //   * Otherwise lanes run consecutive jobs (job = tid % njobs) with the odd
 //     pitch MP = M + 1, and each exchange is bracketed by block barriers.
-// Row njobs is a spare: the threads beyond the njobs*T that carry the
-// transforms run the same schedule on it, so no transform slot is predicated.
+// Threads beyond the njobs*T that carry the transforms: with wave-local rows
+// they sit in the last wave next to active lanes and skip the transform phases
+// as a whole (one branch, no per-slot predicates, no barrier inside); otherwise
+// they run the same schedule on a spare row njobs, so no slot is predicated.
 // Staging and the final store move whole rows, one per wave at a time.
 // The table W_NX^k, k < NX, is copied to LDS once per block: the real<->half-
 // complex pairing reads k < M, the x transforms (length M, stride 2) the rest.
-// Dynamic LDS: ((njobs+1)*MP + NX) complex + (njobs+1) reals + 16 doubles.
+// Dynamic LDS: (rows*MP + NX) complex + (njobs+1) reals + 16 doubles, rows =
+// njobs (+1 for the spare row without wave-local rows).
 // ---------------------------------------------------------------------------
 // Threads per x-line block: FP32 blocks may use 1024 threads, which a 512^3 grid
 // with 37 fields needs (37 transforms x 16 threads); FP64 blocks stop at 512.
@@ -218,6 +221,7 @@ constexpr int kXlineRows = 2;
 // 192^3, so it keeps block barriers, as does 80 (T = 10).
 __host__ __device__ constexpr bool xline_wave_local(int M) { return 64 % plan_T(M) == 0 && plan_np(M) <= 2; }
 __host__ __device__ constexpr int xline_xf(int M) { return xline_wave_local(M) ? (XF_WAVE | XF_SWZ) : 0; }
+__host__ __device__ constexpr int xline_rows(int M, int njobs) { return njobs + (xline_wave_local(M) ? 0 : 1); }
 // Row pitch in elements of cbytes bytes: the bank-model choices for FP32
 // elements; FP64 rows take the smallest padded pitch (at 512 points they must
 // fit 33 rows of 16-byte elements in LDS).
@@ -257,9 +261,10 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const int F = XONLY ? 1 : g.F;
   const int njobs = F * LINES;
   c2<R>* sh = reinterpret_cast<c2<R>*>(smem);
-  c2<R>* shw = sh + (long)(njobs + 1) * MP;  // W_NX^k, k < NX (row njobs: idle threads' row)
+  const int rows = xline_rows(M, njobs);
+  c2<R>* shw = sh + (long)rows * MP;  // W_NX^k, k < NX
   R* shn = reinterpret_cast<R*>(shw + NX);
-  double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)(njobs + 1) * MP + NX) +
+  double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)rows * MP + NX) +
                                                     sizeof(R) * (njobs + 1)) + 15) & ~(size_t)15));
   const int nt = blockDim.x, nw = nt >> 6;
   constexpr int KPL = (M + 63) / 64;  // row elements per lane when a wave moves a row
@@ -330,6 +335,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     }
     __syncthreads();
     // 2. C2R pre-processing straight into the transform's input registers
+    if (!WX || act) {
 #pragma unroll
     for (int s = 0; s < kInSlots<M>; ++s) {
       if (!rfft_in_valid<M>(t, s)) continue;
@@ -348,6 +354,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
       if (rfft_out_valid<M>(t, s)) col[lds_swz<XF>(rfft_out_idx<M>(t, s))] = v[s];
+    }
     __syncthreads();
     // 4. pointwise: point x of line l is component x%2 of element x/2 of row
     //    f*LINES + l.  When the group has more points than the block has threads
@@ -388,6 +395,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     }
     __syncthreads();
     // 5. forward x transform
+    if (!WX || act) {
 #pragma unroll
     for (int s = 0; s < kInSlots<M>; ++s)
       if (rfft_in_valid<M>(t, s)) v[s] = col[lds_swz<XF>(rfft_in_idx<M>(t, s))];
@@ -395,6 +403,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
 #pragma unroll
     for (int s = 0; s < kOutSlots<M>; ++s)
       if (rfft_out_valid<M>(t, s)) col[lds_swz<XF>(rfft_out_idx<M>(t, s))] = v[s];
+    }
     __syncthreads();
     // 6. R2C post-processing and store (coalesced along k), one row per wave
     const R hs = R(0.5) * scale;
