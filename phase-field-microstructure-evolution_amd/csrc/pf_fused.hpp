@@ -228,14 +228,14 @@ __host__ __device__ constexpr int xline_rows(int M, int njobs) { return njobs + 
 __host__ __device__ constexpr int xline_row_pitch(int M, int cbytes) {
   if (!xline_wave_local(M)) return M | 1;
   if (cbytes != 8) return M + M / 16 + 1;
-  switch (M) {  // bank-model choices, >= M + M / 16
+  switch (M) {  // smallest of the bank model's best choices, >= M + M / 16
     case 24: return 26;
-    case 32: return 38;
-    case 48: return 60;
+    case 32: return 34;
+    case 48: return 52;
     case 64: return 72;
     case 96: return 104;
-    case 128: return 152;
-    case 256: return 304;
+    case 128: return 136;
+    case 256: return 272;
     default: return M + M / 16 + 1;
   }
 }
