@@ -87,6 +87,16 @@ __host__ __device__ constexpr int plan_alt(int N, int V, int p) {
       default: return 0;
     }
   }
+  // V = 3: plans of up to 64 threads for one line done by a whole wave (the x
+  // line's job that is alone in the block's last wave)
+  if (V == 3) {
+    switch (N) {
+      case 64: { const int r[6] = {2, 2, 2, 2, 2, 2}; return p < 6 ? r[p] : 1; }
+      case 128: { const int r[7] = {2, 2, 2, 2, 2, 2, 2}; return p < 7 ? r[p] : 1; }
+      case 256: { const int r[4] = {4, 4, 4, 4}; return p < 4 ? r[p] : 1; }
+      default: return 0;
+    }
+  }
   if (V == 2) {
     switch (N) {
       case 64: { const int r[2] = {4, 16}; return p < 2 ? r[p] : 1; }

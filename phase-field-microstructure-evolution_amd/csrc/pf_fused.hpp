@@ -199,6 +199,8 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
 // they sit in the last wave next to active lanes and skip the transform phases
 // as a whole (one branch, no per-slot predicates, no barrier inside); otherwise
 // they run the same schedule on a spare row njobs, so no slot is predicated.
+// A last wave that holds a single job hands it to all its lanes (plan variant
+// 3, where the length has one).
 // Staging and the final store move whole rows, one per wave at a time.
 // The table W_NX^k, k < NX, is copied to LDS once per block: the real<->half-
 // complex pairing reads k < M, the x transforms (length M, stride 2) the rest.
@@ -240,6 +242,50 @@ __host__ __device__ constexpr int xline_row_pitch(int M, int cbytes) {
   }
 }
 
+// Plan variant 3 (radix 2 at 64 and 128 points, radix 4 at 256): one line done
+// by a whole wave.  When njobs = 1 mod 64 / T, the block's last wave carries one
+// job on T of its lanes; with this plan its 64 lanes share that job instead and
+// the wave issues far fewer transform instructions (at 256^3 x 17, 17 jobs of 8
+// threads: x-line kernel 0.577 -> 0.532 ms).
+__host__ __device__ constexpr bool xline_has_wide(int M) {
+  return xline_wave_local(M) && plan_T(M) < 64 && (M == 64 || M == 128 || M == 256);
+}
+
+// Phases 2 and 3 for one job with plan V: the C2R pre-processing straight into
+// the transform's input registers, the inverse x transform, and the real pairs
+// z[n] = (x[2n], x[2n+1]) written back into the row.  No barrier before the
+// write-back: the last Stockham pass leaves each thread's outputs at exactly the
+// elements it loaded as that pass's inputs (out_idx = in_idx when j < NB = LS),
+// so no thread overwrites an element another thread has yet to read.
+template <typename R, int M, int V, int XF>
+__device__ __forceinline__ void xline_c2r(c2<R>* v, c2<R>* col, const c2<R>* shw, const R* xn, int t) {
+#pragma unroll
+  for (int s = 0; s < kInSlots<M, V>; ++s) {
+    if (!rfft_in_valid<M, V>(t, s)) continue;
+    const int k = rfft_in_idx<M, V>(t, s);
+    // k = 0 pairs with the Nyquist entry (its c2r_pre value is not used)
+    const c2<R> pk = c2r_pre<R>(col[lds_swz<XF>(k)], col[lds_swz<XF>((M - k) % M)], shw[k]);
+    const R a0 = col[0].x;
+    v[s] = (k == 0) ? mk<R>(a0 + *xn, a0 - *xn) : pk;
+  }
+  rfft<R, M, true, 1, V, XF>(v, t, true, col, shw, 2);
+#pragma unroll
+  for (int s = 0; s < kOutSlots<M, V>; ++s)
+    if (rfft_out_valid<M, V>(t, s)) col[lds_swz<XF>(rfft_out_idx<M, V>(t, s))] = v[s];
+}
+// Phase 5 for one job with plan V: the forward x transform in place (write-back
+// without a barrier, as above).
+template <typename R, int M, int V, int XF>
+__device__ __forceinline__ void xline_r2c(c2<R>* v, c2<R>* col, const c2<R>* shw, int t) {
+#pragma unroll
+  for (int s = 0; s < kInSlots<M, V>; ++s)
+    if (rfft_in_valid<M, V>(t, s)) v[s] = col[lds_swz<XF>(rfft_in_idx<M, V>(t, s))];
+  rfft<R, M, false, 1, V, XF>(v, t, true, col, shw, 2);
+#pragma unroll
+  for (int s = 0; s < kOutSlots<M, V>; ++s)
+    if (rfft_out_valid<M, V>(t, s)) col[lds_swz<XF>(rfft_out_idx<M, V>(t, s))] = v[s];
+}
+
 // BATCH: ngroups counts the line groups of every member in mt (member m owns
 // groups m*gpm .. m*gpm + gpm - 1, gpm = plane / LINES), each taking its banks
 // and parameters from its entry; wm, wn, u are unused; no diagnostics.
@@ -253,9 +299,12 @@ __global__ void __launch_bounds__(kXlineMaxThreads<R>) __attribute__((amdgpu_wav
 k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __restrict__ mobd,
         const c2<R>* __restrict__ tw, double* __restrict__ part, long ngroups, int diag, int mob,
         const Member<R>* __restrict__ mt) {
-  constexpr int M = NX / 2, T = plan_T(M), E = plan_E(M);
+  constexpr int M = NX / 2, T = plan_T(M);
   constexpr int MP = xline_row_pitch(M, sizeof(c2<R>)), XF = xline_xf(M);
-  constexpr bool WX = xline_wave_local(M);
+  constexpr bool WX = xline_wave_local(M), HAVEW = xline_has_wide(M);
+  // the wide plan's variant (3) and threads; the T-thread plan stands in when
+  // this length has no wide plan, and that branch is never taken then
+  constexpr int WV = HAVEW ? 3 : 0, TW = plan_T(M, WV), JW = HAVEW ? 64 / T : 1;
   static_assert(last_pass_in_place<M>(), "the write-backs after the x transforms rely on it");
   extern __shared__ unsigned char smem[];
   const int F = XONLY ? 1 : g.F;
@@ -272,7 +321,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const R scale = (R)u.invP;
   double a_ch = 0.0, a_eta = 0.0;
   R cmax = 0, cmin = 1, pmin = 1, p3 = 0;
-  c2<R> v[E];
+  c2<R> v[plan_E(M) > plan_E(M, WV) ? plan_E(M) : plan_E(M, WV)];
 
   const long lb = xcd_block(blockIdx.x, gridDim.x);
   const long chunk = (ngroups + gridDim.x - 1) / gridDim.x;
@@ -293,6 +342,10 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     __builtin_assume(t >= 0 && t < T);
     const int row = act ? job : njobs;
     c2<R>* col = sh + row * MP;
+    // the lone job of the last wave (njobs = 1 mod 64 / T) goes to the whole
+    // wave; the condition is uniform over the wave (a scalar branch)
+    const bool wide = HAVEW && njobs % JW == 1 &&
+                      (__builtin_amdgcn_readfirstlane(tid) >> 6) == (njobs - 1) / JW;
     long L0 = lg * LINES, m = 0;
     c2<R>* wmm = wm;
     c2<R>* wnn = wn;
@@ -334,26 +387,11 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       shn[i] = wnn[(long)f * g.plane + L0 + l].x;
     }
     __syncthreads();
-    // 2. C2R pre-processing straight into the transform's input registers
-    if (!WX || act) {
-#pragma unroll
-    for (int s = 0; s < kInSlots<M>; ++s) {
-      if (!rfft_in_valid<M>(t, s)) continue;
-      const int k = rfft_in_idx<M>(t, s);
-      // k = 0 pairs with the Nyquist entry (its c2r_pre value is not used)
-      const c2<R> pk = c2r_pre<R>(col[lds_swz<XF>(k)], col[lds_swz<XF>((M - k) % M)], shw[k]);
-      const R a0 = col[0].x, xn = shn[row];
-      v[s] = (k == 0) ? mk<R>(a0 + xn, a0 - xn) : pk;
-    }
-    // 3. inverse x transform; real pairs z[n] = (x[2n], x[2n+1]) back into the row.
-    //    No barrier before the write-back: the last Stockham pass leaves each
-    //    thread's outputs at exactly the elements it loaded as that pass's inputs
-    //    (out_idx = in_idx when j < NB = LS), so no thread overwrites an element
-    //    another thread has yet to read.
-    rfft<R, M, true, 1, 0, XF>(v, t, true, col, shw, 2);
-#pragma unroll
-    for (int s = 0; s < kOutSlots<M>; ++s)
-      if (rfft_out_valid<M>(t, s)) col[lds_swz<XF>(rfft_out_idx<M>(t, s))] = v[s];
+    // 2-3. C2R pre-processing and inverse x transform of each job
+    if (wide) {
+      if (TW == 64 || lane < TW) xline_c2r<R, M, WV, XF>(v, sh + (njobs - 1) * MP, shw, shn + njobs - 1, lane);
+    } else if (!WX || act) {
+      xline_c2r<R, M, 0, XF>(v, col, shw, shn + row, t);
     }
     __syncthreads();
     // 4. pointwise: point x of line l is component x%2 of element x/2 of row
@@ -394,15 +432,11 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       }
     }
     __syncthreads();
-    // 5. forward x transform
-    if (!WX || act) {
-#pragma unroll
-    for (int s = 0; s < kInSlots<M>; ++s)
-      if (rfft_in_valid<M>(t, s)) v[s] = col[lds_swz<XF>(rfft_in_idx<M>(t, s))];
-    rfft<R, M, false, 1, 0, XF>(v, t, true, col, shw, 2);  // write-back without a barrier, as in 3.
-#pragma unroll
-    for (int s = 0; s < kOutSlots<M>; ++s)
-      if (rfft_out_valid<M>(t, s)) col[lds_swz<XF>(rfft_out_idx<M>(t, s))] = v[s];
+    // 5. forward x transform of each job
+    if (wide) {
+      if (TW == 64 || lane < TW) xline_r2c<R, M, WV, XF>(v, sh + (njobs - 1) * MP, shw, lane);
+    } else if (!WX || act) {
+      xline_r2c<R, M, 0, XF>(v, col, shw, t);
     }
     __syncthreads();
     // 6. R2C post-processing and store (coalesced along k), one row per wave

@@ -111,6 +111,7 @@ int main() {
   RUN(48) RUN(64) RUN(80) RUN(96) RUN(128) RUN(160) RUN(192) RUN(256) RUN(512)
   RUNV(64, 1) RUNV(128, 1) RUNV(160, 1) RUNV(192, 1) RUNV(256, 1) RUNV(512, 1)
   RUNV(64, 2) RUNV(128, 2) RUNV(160, 2) RUNV(192, 2) RUNV(256, 2) RUNV(512, 2)
+  RUNV(64, 3) RUNV(128, 3) RUNV(256, 3)
   RUNS(4) RUNS(6) RUNS(8) RUNS(12) RUNS(16) RUNS(24) RUNS(32) RUNS(48) RUNS(64) RUNS(96) RUNS(128) RUNS(256)
   return 0;
 }

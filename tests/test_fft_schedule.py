@@ -53,6 +53,15 @@ def test_swizzled_exchange_matches_dft(emul_output, n):
     assert perm == "1"
 
 
+@pytest.mark.parametrize("n", [64, 128, 256])
+def test_wave_wide_plans_match_dft(emul_output, n):
+    """Plan variant 3: the x line's lone job in a block's last wave, transformed
+    by up to 64 threads (radix 2 at 64 and 128 points, radix 4 at 256)."""
+    _, _, npass, ef, ei, inplace, nthreads = emul_output[(n, 3)]
+    assert float(ef) < 1e-13 * n and float(ei) < 1e-13 * n
+    assert inplace == "1" and int(nthreads) <= 64
+
+
 def test_variants_differ_from_the_default_plan(emul_output):
     for n, v in VARIANTS:
         assert emul_output[(n, v)][6] != emul_output[(n, 0)][6] or \

@@ -27,10 +27,12 @@ def energy_scale(o, k):
     return max(abs(o[k]), 1.0)
 
 
-def _compare(pf, oracle, shape, var, nseeds, nsteps, mode, params, tol_state, tol_e):
+def _compare(pf, oracle, shape, var, nseeds, nsteps, mode, params, tol_state, tol_e, tuning=None):
     nx, ny, nz = shape
     phi, con = pf.ic_voronoi(nx, ny, nz, var, nseeds)
     with pf.Simulation(pf.make_params(**params), nx, ny, nz, var, mode) as sim:
+        if tuning:
+            sim.set_tuning(**tuning)
         sim.put_state(phi, con)
         dg = sim.step(nsteps, diag_every=2)
         gphi, gcon = sim.get_state()
@@ -276,3 +278,16 @@ def test_axis_lengths_vs_oracle(pf, oracle, shape, mode, tol_s, tol_e):
 @pytest.mark.parametrize("shape", [(160, 8, 8), (12, 96, 8), (8, 12, 192)], ids=lambda s: "x".join(map(str, s)))
 def test_axis_lengths_faithful_vs_oracle(pf, oracle, shape):
     _compare(pf, oracle, shape, 3, 5, 4, "fp64_faithful", dict(VOR, D_gb=0.25), 1e-13, 1e-13)
+
+
+# One line per x-line block with fields = 1 mod 64 / T (T threads per line: 8 at
+# 64 and 128 points, 16 at 256): the block's last wave holds one job, which its
+# 64 lanes transform together with the radix-2 / radix-4 plan.  The mobility
+# chain's one-field blocks (D_gb != 0) take that plan as well.
+LONE_JOB = [((128, 8, 8), 8), ((256, 8, 8), 8), ((256, 8, 8), 16), ((512, 8, 8), 4), ((256, 8, 8), 4)]
+
+
+@pytest.mark.parametrize("shape,var", LONE_JOB, ids=lambda s: "x".join(map(str, s)) if isinstance(s, tuple) else str(s))
+@pytest.mark.parametrize("mode,tol_s,tol_e", [("fp64_reduced", 1e-13, 1e-13), ("fp32_mixed", 2e-6, 2e-6)])
+def test_lone_xline_job_vs_oracle(pf, oracle, shape, var, mode, tol_s, tol_e):
+    _compare(pf, oracle, shape, var, 6, 4, mode, dict(VOR, D_gb=0.25), tol_s, tol_e, tuning=dict(xline_lines=1))
