@@ -201,11 +201,13 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
 // they run the same schedule on a spare row njobs, so no slot is predicated.
 // A last wave that holds a single job hands it to all its lanes (plan variant
 // 3, where the length has one).
-// Staging and the final store move whole rows, one per wave at a time.
-// The table W_NX^k, k < NX, is copied to LDS once per block: the real<->half-
-// complex pairing reads k < M, the x transforms (length M, stride 2) the rest.
-// Dynamic LDS: (rows*MP + NX) complex + (njobs+1) reals + 16 doubles, rows =
-// njobs (+1 for the spare row without wave-local rows).
+// Staging and the final store move whole rows, one per wave at a time; they
+// also do the real <-> half-complex pairing, a lane taking elements k and M - k
+// together (xline_partner), so the transforms read and write their rows only.
+// The table W_NX^k, k < NX, is copied to LDS once per block: the pairing reads
+// k < M / 2, the x transforms (length M, stride 2) the rest.
+// Dynamic LDS: (rows*MP + NX) complex + 16 doubles, rows = njobs (+1 for the
+// spare row without wave-local rows).
 // ---------------------------------------------------------------------------
 // Threads per x-line block: FP32 blocks may use 1024 threads, which a 512^3 grid
 // with 37 fields needs (37 transforms x 16 threads); FP64 blocks stop at 512.
@@ -251,12 +253,9 @@ __host__ __device__ constexpr bool xline_has_wide(int M) {
   return xline_wave_local(M) && plan_T(M) < 64 && (M == 64 || M == 128 || M == 256);
 }
 
-// Phases 2 and 3 for one job with plan V: the C2R pre-processing straight into
-// the transform's input registers, the inverse x transform, and the real pairs
-// z[n] = (x[2n], x[2n+1]) written back into the row.  No barrier before the
-// write-back: the last Stockham pass leaves each thread's outputs at exactly the
-// elements it loaded as that pass's inputs (out_idx = in_idx when j < NB = LS),
-// so no thread overwrites an element another thread has yet to read.
+// Phases 2 and 3 when the staging did not pair (xline_pair_rows false): the C2R
+// pre-processing from the raw row straight into the transform's input
+// registers, then the inverse transform in place, as xline_fft.
 template <typename R, int M, int V, int XF>
 __device__ __forceinline__ void xline_c2r(c2<R>* v, c2<R>* col, const c2<R>* shw, const R* xn, int t) {
 #pragma unroll
@@ -273,17 +272,41 @@ __device__ __forceinline__ void xline_c2r(c2<R>* v, c2<R>* col, const c2<R>* shw
   for (int s = 0; s < kOutSlots<M, V>; ++s)
     if (rfft_out_valid<M, V>(t, s)) col[lds_swz<XF>(rfft_out_idx<M, V>(t, s))] = v[s];
 }
-// Phase 5 for one job with plan V: the forward x transform in place (write-back
-// without a barrier, as above).
-template <typename R, int M, int V, int XF>
-__device__ __forceinline__ void xline_r2c(c2<R>* v, c2<R>* col, const c2<R>* shw, int t) {
+// One job's x transform with plan V, in place in its LDS row (phases 3 and 5).
+// No barrier before the write-back: the last Stockham pass leaves each
+// thread's outputs at exactly the elements it loaded as that pass's inputs
+// (out_idx = in_idx when j < NB = LS), so no thread overwrites an element
+// another thread has yet to read.
+template <typename R, int M, int V, int XF, bool INV>
+__device__ __forceinline__ void xline_fft(c2<R>* v, c2<R>* col, const c2<R>* shw, int t) {
 #pragma unroll
   for (int s = 0; s < kInSlots<M, V>; ++s)
     if (rfft_in_valid<M, V>(t, s)) v[s] = col[lds_swz<XF>(rfft_in_idx<M, V>(t, s))];
-  rfft<R, M, false, 1, V, XF>(v, t, true, col, shw, 2);
+  rfft<R, M, INV, 1, V, XF>(v, t, true, col, shw, 2);
 #pragma unroll
   for (int s = 0; s < kOutSlots<M, V>; ++s)
     if (rfft_out_valid<M, V>(t, s)) col[lds_swz<XF>(rfft_out_idx<M, V>(t, s))] = v[s];
+}
+
+// The real <-> half-complex pairing couples element k of a row with element
+// M - k.  A lane takes both: pair p < M / 2 is (p, M - p), and p = 0 stands for
+// the two self-coupled entries, 0 (with the Nyquist value) and M / 2.  Each
+// element is then read once and the twiddle of M - p follows from that of p,
+// W_NX^(M-p) = -conj(W_NX^p) (bit for bit: the table is built by exact quadrant
+// symmetries); W_NX^(M/2) = (-0, -1) exactly.
+template <int M> __device__ __forceinline__ int xline_partner(int p) { return p ? M - p : M / 2; }
+// Where the pairings happen: in the staging and the final store (each wave's
+// lanes pair the row they move), or in the transform threads' loads and the
+// store's lanes, from the raw rows (three LDS reads per element, the Nyquist
+// values in LDS too).  256-point rows are paired by the moving waves (512^3 x
+// 37: x line 15.4 -> 14.7 ms; the kernel is bound by LDS traffic there).  At
+// 128 points and fewer, pairing in the staging made the x line slower (256^3 x
+// 17: 0.532 -> 0.553 ms; 128^3 x 9: 0.061 -> 0.078 ms, half the lanes idle in a
+// 64-point row's pairs) and pairing in the store alone moved it by less than
+// the run-to-run spread.
+__host__ __device__ constexpr bool xline_pair_rows(int M) { return M >= 256; }
+template <typename R> __device__ __forceinline__ c2<R> xline_mirror_tw(c2<R> w, int p) {
+  return p ? mk<R>(-w.x, w.y) : mk<R>(R(-0.0), R(-1));
 }
 
 // BATCH: ngroups counts the line groups of every member in mt (member m owns
@@ -311,13 +334,16 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const int njobs = F * LINES;
   c2<R>* sh = reinterpret_cast<c2<R>*>(smem);
   const int rows = xline_rows(M, njobs);
+  constexpr bool PP = xline_pair_rows(M);
   c2<R>* shw = sh + (long)rows * MP;  // W_NX^k, k < NX
-  R* shn = reinterpret_cast<R*>(shw + NX);
+  R* shn = reinterpret_cast<R*>(shw + NX);  // Nyquist values (without PP)
   double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)rows * MP + NX) +
-                                                    sizeof(R) * (njobs + 1)) + 15) & ~(size_t)15));
+                                                    (PP ? 0 : sizeof(R) * (njobs + 1))) + 15) & ~(size_t)15));
   const int nt = blockDim.x, nw = nt >> 6;
-  constexpr int KPL = (M + 63) / 64;  // row elements per lane when a wave moves a row
+  constexpr int PPL = (M / 2 + 63) / 64;  // element pairs per lane when a wave moves a row
+  constexpr int KPL = (M + 63) / 64;      // row elements per lane when a wave moves a row
   for (int k = threadIdx.x; k < NX; k += nt) shw[k] = tw[k];
+  __syncthreads();  // the staging of the first line group reads the table
   const R scale = (R)u.invP;
   double a_ch = 0.0, a_eta = 0.0;
   R cmax = 0, cmin = 1, pmin = 1, p3 = 0;
@@ -357,41 +383,89 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       wnn = mt[m].Wn;
     }
     const UPar& uu = member_u<R>(u, mt, m, BATCH);
-    // 1. half lines of every field -> LDS rows: wave w stages rows w, w + nw, ...
-    //    (coalesced along k), kXlineRows rows per batch in flight
-    for (int jb0 = wv; jb0 < njobs; jb0 += kXlineRows * nw) {
-      c2<R> tmp[kXlineRows][KPL];
+    if constexpr (PP) {
+      // 1. half lines of every field -> LDS rows, with the C2R pre-processing:
+      //    wave w stages rows w, w + nw, ... (coalesced along k, and along M - k
+      //    in descending order), kXlineRows rows per batch in flight
+      for (int jb0 = wv; jb0 < njobs; jb0 += kXlineRows * nw) {
+        c2<R> ta[kXlineRows][PPL], tb[kXlineRows][PPL];
+        R xn[kXlineRows];
 #pragma unroll
-      for (int r = 0; r < kXlineRows; ++r) {
-        const int jb = jb0 + r * nw;
-        if (jb < njobs) {
-          const int f = jb / LINES, l = jb - f * LINES;
-          const c2<R>* src = wmm + (long)f * g.hmain + (L0 + l) * M;
+        for (int r = 0; r < kXlineRows; ++r) {
+          const int jb = jb0 + r * nw;
+          if (jb < njobs) {
+            const int f = jb / LINES, l = jb - f * LINES;
+            const c2<R>* src = wmm + (long)f * g.hmain + (L0 + l) * M;
 #pragma unroll
-          for (int kk = 0; kk < KPL; ++kk)
-            if (M % 64 == 0 || lane + 64 * kk < M) tmp[r][kk] = ld_nt<R>(src + lane + 64 * kk);
+            for (int j = 0; j < PPL; ++j) {
+              const int p = lane + 64 * j;
+              if ((M / 2) % 64 == 0 || p < M / 2) {
+                ta[r][j] = ld_nt<R>(src + p);
+                tb[r][j] = ld_nt<R>(src + xline_partner<M>(p));
+              }
+            }
+            xn[r] = (lane == 0) ? wnn[(long)f * g.plane + L0 + l].x : R(0);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < kXlineRows; ++r) {
+          const int jb = jb0 + r * nw;
+          if (jb < njobs) {
+            c2<R>* rw = sh + jb * MP;
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+              const int p = lane + 64 * j;
+              if ((M / 2) % 64 == 0 || p < M / 2) {
+                const c2<R> a = ta[r][j], b = tb[r][j], w = shw[p];
+                rw[lds_swz<XF>(p)] = p ? c2r_pre<R>(a, b, w) : mk<R>(a.x + xn[r], a.x - xn[r]);
+                rw[lds_swz<XF>(xline_partner<M>(p))] = c2r_pre<R>(b, p ? a : b, xline_mirror_tw<R>(w, p));
+              }
+            }
+          }
         }
       }
+    } else {
+      // 1. half lines of every field -> LDS rows: wave w stages rows w, w + nw,
+      //    ... (coalesced along k), kXlineRows rows per batch in flight
+      for (int jb0 = wv; jb0 < njobs; jb0 += kXlineRows * nw) {
+        c2<R> tmp[kXlineRows][KPL];
 #pragma unroll
-      for (int r = 0; r < kXlineRows; ++r) {
-        const int jb = jb0 + r * nw;
-        if (jb < njobs) {
+        for (int r = 0; r < kXlineRows; ++r) {
+          const int jb = jb0 + r * nw;
+          if (jb < njobs) {
+            const int f = jb / LINES, l = jb - f * LINES;
+            const c2<R>* src = wmm + (long)f * g.hmain + (L0 + l) * M;
 #pragma unroll
-          for (int kk = 0; kk < KPL; ++kk)
-            if (M % 64 == 0 || lane + 64 * kk < M) sh[jb * MP + lds_swz<XF>(lane + 64 * kk)] = tmp[r][kk];
+            for (int kk = 0; kk < KPL; ++kk)
+              if (M % 64 == 0 || lane + 64 * kk < M) tmp[r][kk] = ld_nt<R>(src + lane + 64 * kk);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < kXlineRows; ++r) {
+          const int jb = jb0 + r * nw;
+          if (jb < njobs) {
+#pragma unroll
+            for (int kk = 0; kk < KPL; ++kk)
+              if (M % 64 == 0 || lane + 64 * kk < M) sh[jb * MP + lds_swz<XF>(lane + 64 * kk)] = tmp[r][kk];
+          }
         }
       }
-    }
-    for (int i = tid; i < njobs; i += nt) {
-      const int f = i / LINES, l = i - f * LINES;
-      shn[i] = wnn[(long)f * g.plane + L0 + l].x;
+      for (int i = tid; i < njobs; i += nt) {
+        const int f = i / LINES, l = i - f * LINES;
+        shn[i] = wnn[(long)f * g.plane + L0 + l].x;
+      }
     }
     __syncthreads();
-    // 2-3. C2R pre-processing and inverse x transform of each job
+    // 2-3. (C2R pre-processing and) inverse x transform of each job; real
+    //      pairs z[n] = (x[2n], x[2n+1])
     if (wide) {
-      if (TW == 64 || lane < TW) xline_c2r<R, M, WV, XF>(v, sh + (njobs - 1) * MP, shw, shn + njobs - 1, lane);
+      if (TW == 64 || lane < TW) {
+        if constexpr (PP) xline_fft<R, M, WV, XF, true>(v, sh + (njobs - 1) * MP, shw, lane);
+        else xline_c2r<R, M, WV, XF>(v, sh + (njobs - 1) * MP, shw, shn + njobs - 1, lane);
+      }
     } else if (!WX || act) {
-      xline_c2r<R, M, 0, XF>(v, col, shw, shn + row, t);
+      if constexpr (PP) xline_fft<R, M, 0, XF, true>(v, col, shw, t);
+      else xline_c2r<R, M, 0, XF>(v, col, shw, shn + row, t);
     }
     __syncthreads();
     // 4. pointwise: point x of line l is component x%2 of element x/2 of row
@@ -434,22 +508,36 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     __syncthreads();
     // 5. forward x transform of each job
     if (wide) {
-      if (TW == 64 || lane < TW) xline_r2c<R, M, WV, XF>(v, sh + (njobs - 1) * MP, shw, lane);
+      if (TW == 64 || lane < TW) xline_fft<R, M, WV, XF, false>(v, sh + (njobs - 1) * MP, shw, lane);
     } else if (!WX || act) {
-      xline_r2c<R, M, 0, XF>(v, col, shw, t);
+      xline_fft<R, M, 0, XF, false>(v, col, shw, t);
     }
     __syncthreads();
-    // 6. R2C post-processing and store (coalesced along k), one row per wave
+    // 6. R2C post-processing and store, one row per wave: coalesced along k, or
+    //    with the pairs (p, M - p) of 1 per lane, along k and M - k
     const R hs = R(0.5) * scale;
     for (int jb = wv; jb < njobs; jb += nw) {
       const int f = jb / LINES, l = jb - f * LINES;
       c2<R>* dst = wmm + (long)f * g.hmain + (L0 + l) * M;
       const c2<R>* rw = sh + jb * MP;
+      if constexpr (PP) {
 #pragma unroll
-      for (int kk = 0; kk < KPL; ++kk) {
-        const int k = lane + 64 * kk;
-        if (M % 64 == 0 || k < M)
-          st_nt<R>(dst + k, r2c_post<R>(rw[lds_swz<XF>(k)], rw[lds_swz<XF>((M - k) % M)], shw[k], hs));
+        for (int j = 0; j < PPL; ++j) {
+          const int p = lane + 64 * j;
+          if ((M / 2) % 64 == 0 || p < M / 2) {
+            const int q = xline_partner<M>(p);
+            const c2<R> a = rw[lds_swz<XF>(p)], b = rw[lds_swz<XF>(q)], w = shw[p];
+            st_nt<R>(dst + p, r2c_post<R>(a, p ? b : a, w, hs));
+            st_nt<R>(dst + q, r2c_post<R>(b, p ? a : b, xline_mirror_tw<R>(w, p), hs));
+          }
+        }
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < KPL; ++kk) {
+          const int k = lane + 64 * kk;
+          if (M % 64 == 0 || k < M)
+            st_nt<R>(dst + k, r2c_post<R>(rw[lds_swz<XF>(k)], rw[lds_swz<XF>((M - k) % M)], shw[k], hs));
+        }
       }
     }
     for (int i = tid; i < njobs; i += nt) {

@@ -844,7 +844,8 @@ bool xline_fits(const Geo& g, long* nt_out, size_t* shb_out) {
   const long jobs = (long)F * LINES;
   const long MP = xline_row_pitch(M, sizeof(c2<R>));
   long need = std::max<long>(64, (jobs * T + 63) / 64 * 64);
-  const size_t body = sizeof(c2<R>) * ((size_t)xline_rows(M, (int)jobs) * MP + NX) + sizeof(R) * (size_t)(jobs + 1);
+  const size_t body = sizeof(c2<R>) * ((size_t)xline_rows(M, (int)jobs) * MP + NX) +
+                      (xline_pair_rows(M) ? 0 : sizeof(R) * (size_t)(jobs + 1));
   const size_t shb = ((body + 15) & ~(size_t)15) + 16 * sizeof(double);
   if (need > kXlineMaxThreads<R> || shb > kXlineMaxLds) return false;
   *nt_out = need;
