@@ -134,6 +134,13 @@ def test_fp32_long_x_lines_many_fields(pf, oracle):
     _compare(pf, oracle, (512, 16, 16), 36, 40, 6, "fp32_mixed", dict(VOR, D_gb=0.0), 2e-6, 2e-6)
 
 
+def test_fp32_long_x_lines_several_groups_per_block(pf, oracle):
+    """As above with 1024 lines over 256 blocks: each block takes four line groups
+    and loads the next group's rows while it works on the current one."""
+    _compare(pf, oracle, (512, 32, 32), 36, 40, 4, "fp32_mixed", dict(VOR, D_gb=0.25), 2e-6, 2e-6,
+             tuning=dict(xline_grid=256))
+
+
 def test_fp64_reduced_too_many_fields_is_an_error(pf):
     """FP64 x-line blocks stop at 512 threads: 37 fields of 512-point lines need 592."""
     phi, con = pf.ic_voronoi(512, 16, 16, 36, 40)
