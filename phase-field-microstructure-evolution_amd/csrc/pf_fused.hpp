@@ -256,14 +256,29 @@ __host__ __device__ constexpr bool xline_has_wide(int M) {
 // Phases 2 and 3 when the staging did not pair (xline_pair_rows false): the C2R
 // pre-processing from the raw row straight into the transform's input
 // registers, then the inverse transform in place, as xline_fft.
+// The partner of slot s's element k = t + T s (pass 0 has one butterfly per
+// thread, NB = T) is (M - k) mod M = u + T c, u = T - t, c = M / T - 1 - s; for
+// k = 0 this reads element M instead, one past the row (inside the LDS tile,
+// and not used).  In a swizzled row (e + e / 16) its address is a per-lane base,
+// one for each residue of c mod 16 / T, plus a compile-time offset; computed
+// per slot the swizzle took six or seven VALU instructions a slot.
 template <typename R, int M, int V, int XF>
 __device__ __forceinline__ void xline_c2r(c2<R>* v, c2<R>* col, const c2<R>* shw, const R* xn, int t) {
+  constexpr int T = plan_T(M, V);
+  static_assert(PassC<M, 0, V>::NB == T && PassC<M, 0, V>::Q == 1, "pass 0: element t + T s in slot s");
+  constexpr int NR = T < 16 ? 16 / T : 1;
+  const int u = T - t;
+  int mb[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) mb[r] = u + ((u + T * r) >> 4);
 #pragma unroll
   for (int s = 0; s < kInSlots<M, V>; ++s) {
     if (!rfft_in_valid<M, V>(t, s)) continue;
     const int k = rfft_in_idx<M, V>(t, s);
+    const int c = M / T - 1 - s;
+    const int mi = (XF & XF_SWZ) ? mb[c % NR] + T * c + (T < 16 ? c / NR : T * c / 16) : (M - k) % M;
     // k = 0 pairs with the Nyquist entry (its c2r_pre value is not used)
-    const c2<R> pk = c2r_pre<R>(col[lds_swz<XF>(k)], col[lds_swz<XF>((M - k) % M)], shw[k]);
+    const c2<R> pk = c2r_pre<R>(col[lds_swz<XF>(k)], col[mi], shw[k]);
     const R a0 = col[0].x;
     v[s] = (k == 0) ? mk<R>(a0 + *xn, a0 - *xn) : pk;
   }
