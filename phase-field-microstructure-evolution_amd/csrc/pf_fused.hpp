@@ -672,6 +672,14 @@ __device__ __forceinline__ double recip_fast(double x) {
 // FP64.  At 256 the FP32 update takes 165 VGPRs (3 waves) with no spills;
 // capped at 4 waves it spilled 6-25 registers per lane and ran 16% slower.
 template <typename R, bool PARSEVAL, int E = 16> constexpr int kZlineWaves = (sizeof(R) == 4 && E <= 16) ? 3 : 2;
+// 512-point FP32 columns with the 32-thread plans (V = 1, 2: 512-thread blocks,
+// 64 KiB of LDS): capped at 128 VGPRs (4 waves per SIMD) two blocks share a CU,
+// one loading while the other transforms.  They spill 11-15 VGPRs per lane that
+// way and still take 16.3-16.5 ms per update pass at 512^3 x 37, against
+// 21.0-22.3 ms uncapped (one block per CU) and 20.9 ms for the default plan's
+// 1 024-thread blocks.
+template <typename R, int NZ, int V, bool PARSEVAL>
+constexpr int kZlineWavesN = (sizeof(R) == 4 && NZ >= 512 && V > 0) ? 4 : kZlineWaves<R, PARSEVAL, plan_E(NZ, V)>;
 
 // The update coefficients of one parameter set, FP64.
 struct ZCo {
@@ -686,7 +694,7 @@ __device__ __forceinline__ ZCo zco(const UPar& u) {
 // and dc64 are unused.
 template <typename R, int NZ, int ROLE, bool DC64, bool PARSEVAL, bool BATCH = false, int V = 0>
 __global__ void __launch_bounds__(col_tile(NZ, sizeof(c2<R>)) * plan_T(NZ, V))
-__attribute__((amdgpu_waves_per_eu(kZlineWaves<R, PARSEVAL, plan_E(NZ, V)>)))
+__attribute__((amdgpu_waves_per_eu(kZlineWavesN<R, NZ, V, PARSEVAL>)))
 k_zline(Geo g, UPar u, KTab kt, Bank<R> S, Bank<R> W, Bank<R> X, Bank<R> Y,
         double* __restrict__ dc64, double* __restrict__ part, const c2<R>* __restrict__ tw, int f0, int nf,
         long tm, long tn, long run_tiles, const Member<R>* __restrict__ mt) {
