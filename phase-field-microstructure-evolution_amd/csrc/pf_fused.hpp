@@ -201,12 +201,15 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
 // they run the same schedule on a spare row njobs, so no slot is predicated.
 // A last wave that holds a single job hands it to all its lanes (plan variant
 // 3, where the length has one).
-// Staging and the final store move whole rows, one per wave at a time; they
-// also do the real <-> half-complex pairing, a lane taking elements k and M - k
-// together (xline_partner), so the transforms read and write their rows only.
+// Staging and the final store move whole rows, one per wave at a time.  For
+// 256-point rows they also do the real <-> half-complex pairing, a lane taking
+// elements k and M - k together (xline_partner, xline_pair_rows); shorter rows
+// are paired by the transform threads as they load (C2R) and by the store's
+// lanes (R2C), with the Nyquist values staged in LDS.
 // The table W_NX^k, k < NX, is copied to LDS once per block: the pairing reads
-// k < M / 2, the x transforms (length M, stride 2) the rest.
-// Dynamic LDS: (rows*MP + NX) complex + 16 doubles, rows = njobs (+1 for the
+// k < M, the x transforms (length M, stride 2) the rest.
+// Dynamic LDS: (rows*MP + NX) complex (+ njobs + 1 reals for the Nyquist values
+// of rows the staging does not pair) + 16 doubles, rows = njobs (+1 for the
 // spare row without wave-local rows).
 // ---------------------------------------------------------------------------
 // Threads per x-line block: FP32 blocks may use 1024 threads, which a 512^3 grid
