@@ -86,7 +86,10 @@ template <bool MAX> __device__ __forceinline__ double bext(double v, double* red
 // derivative likewise as eta_V (gamma av + delta).  The regrouping moves each
 // derivative by a few ulp of beta (the derivative enters the update multiplied
 // by dt L).  Extremes are tracked in the working precision (exact: they are
-// values of the field); the two energy sums in FP64.
+// values of the field); the two energy sums in FP64.  The matrix grains' part
+// of the bulk energy needs no work per field beyond the sums the derivatives
+// use: sum_i (a_i^2 / 4 - a_i / 2) = num / 4 - sm / 2 and
+// sum_{i<j} a_i a_j = (sm^2 - num) / 2, with sm = sum a_i, num = sum a_i^2.
 template <typename R, int NPT> struct Pts;
 template <typename R> struct Pts<R, 1> {
   using T = R;
@@ -105,14 +108,14 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
   using P = Pts<R, NPT>;
   const R am = (R)u.am, ap = (R)u.ap, cm = (R)u.cm, cp = (R)u.cp, s1g = (R)u.s1g, s2g = (R)u.s2g;
   const int mat = V - 1;
-  R con[NPT], sm[NPT], num[NPT], pair[NPT], fe[NPT], fv[NPT];
+  R con[NPT], sm[NPT], num[NPT], fv[NPT];
   {
     const typename P::T c0 = v[0], f0 = v[V * fs];
 #pragma unroll
     for (int h = 0; h < NPT; ++h) {
       con[h] = P::get(c0, h);
       fv[h] = P::get(f0, h);
-      sm[h] = num[h] = pair[h] = fe[h] = R(0);
+      sm[h] = num[h] = R(0);
     }
   }
 #pragma unroll 2
@@ -121,10 +124,8 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
 #pragma unroll
     for (int h = 0; h < NPT; ++h) {
       const R fi = P::get(f, h), a = fi * fi;
-      if (DIAG) pair[h] += a * sm[h];
       sm[h] += a;
       num[h] += a * a;
-      if (DIAG) fe[h] += a * (R(0.25) * a - R(0.5));
     }
   }
   R alpha[NPT], beta[NPT], gam[NPT], hm[NPT], hp[NPT], fm[NPT], fpp[NPT], av[NPT], out[NPT];
@@ -165,9 +166,11 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
     const R ps = sm[h] + av[h];
     mo[h] = R(2) * (R(1) - ps) * (R)u.Dgb;
     if (DIAG) {
-      fe[h] += av[h] * (R(0.25) * av[h] - R(0.5)) + s1g * pair[h] + s2g * av[h] * sm[h];
+      const R pair = R(0.5) * fma(sm[h], sm[h], -num[h]);
+      const R fe = (R(0.25) * num[h] - R(0.5) * sm[h]) + av[h] * (R(0.25) * av[h] - R(0.5)) + s1g * pair +
+                   s2g * av[h] * sm[h];
       a_ch += (double)(hm[h] * fm[h] + hp[h] * fpp[h]);
-      a_eta += (double)fe[h];
+      a_eta += (double)fe;
       if (fv[h] > p3) p3 = fv[h];
       if (con[h] <= cmin) cmin = con[h];
       if (con[h] > cmax) cmax = con[h];
