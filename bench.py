@@ -179,8 +179,8 @@ def cpu_model() -> str:
 
 def cpu_threads() -> int:
     """The host threads the CPU leg uses: OMP_NUM_THREADS when the environment
-    sets it (the GPU box does: its CPU share per GPU), else every core in this
-    process's affinity mask."""
+    sets it (a host shared between GPUs sets each GPU's CPU share there), else
+    every core in this process's affinity mask."""
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
         return int(env)
