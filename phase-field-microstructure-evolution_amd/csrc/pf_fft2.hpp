@@ -359,10 +359,14 @@ template <int N, int V = 0> struct RPlan {
 };
 
 // One pass's arithmetic on registers: twiddles (p > 0) then radix-r DFTs.
-template <typename R, int N, bool INV, int P, int V = 0>
-__host__ __device__ __forceinline__ void rpass_compute(c2<R>* v, int t, bool act, const c2<R>* __restrict__ tw, int twm) {
+// TWR: the pass has one butterfly per thread and takes its twiddles from twr[m - 1]
+// (registers the caller filled with last_pass_twiddles) instead of the table.
+template <typename R, int N, bool INV, int P, int V = 0, bool TWR = false>
+__host__ __device__ __forceinline__ void rpass_compute(c2<R>* v, int t, bool act, const c2<R>* __restrict__ tw, int twm,
+                                                       const c2<R>* twr = nullptr) {
   using C = PassC<N, P, V>;
   constexpr int r = C::r, NB = C::NB, LS = C::LS, Q = C::Q, T = C::T;
+  static_assert(!TWR || (Q == 1 && LS > 1), "register twiddles: one butterfly per thread, a pass after the first");
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = t + q * T;
@@ -371,7 +375,7 @@ __host__ __device__ __forceinline__ void rpass_compute(c2<R>* v, int t, bool act
       const int k = j % LS;
 #pragma unroll
       for (int m = 1; m < r; ++m) {
-        const c2<R> w = tw[(m * k * (N / (LS * r))) * twm];
+        const c2<R> w = TWR ? twr[m - 1] : tw[(m * k * (N / (LS * r))) * twm];
         v[q * r + m] = INV ? hmulc<R>(v[q * r + m], w) : hmul<R>(v[q * r + m], w);
       }
     }
@@ -437,23 +441,24 @@ __device__ __forceinline__ void rpass_load(c2<R>* v, int t, bool act, const c2<R
 
 // Passes P.. of the transform; each exchange is bracketed by block barriers, or
 // by wave-scope fences when the line belongs to one wave (XF_WAVE).
-template <typename R, int N, bool INV, int ES, int P, int V = 0, int XF = 0>
+// TWR: the last pass's twiddles come from twr (see rpass_compute).
+template <typename R, int N, bool INV, int ES, int P, int V = 0, int XF = 0, bool TWR = false>
 __device__ __forceinline__ void rfft_from(c2<R>* v, int t, bool act, c2<R>* lds, const c2<R>* __restrict__ tw,
-                                          int twm, int es) {
-  rpass_compute<R, N, INV, P, V>(v, t, act, tw, twm);
+                                          int twm, int es, const c2<R>* twr = nullptr) {
+  rpass_compute<R, N, INV, P, V, TWR && (P + 1 == plan_np(N, V))>(v, t, act, tw, twm, twr);
   if constexpr (P + 1 < plan_np(N, V)) {
     xsync<XF>();
     rpass_store<R, N, P, ES, V, XF>(v, t, act, lds, es);
     xsync<XF>();
     rpass_load<R, N, P + 1, ES, V, XF>(v, t, act, lds, es);
-    rfft_from<R, N, INV, ES, P + 1, V, XF>(v, t, act, lds, tw, twm, es);
+    rfft_from<R, N, INV, ES, P + 1, V, XF, TWR>(v, t, act, lds, tw, twm, es, twr);
   }
 }
 // ES = 0: the LDS element stride is the runtime value es.  V: plan variant.
-template <typename R, int N, bool INV, int ES, int V = 0, int XF = 0>
+template <typename R, int N, bool INV, int ES, int V = 0, int XF = 0, bool TWR = false>
 __device__ __forceinline__ void rfft(c2<R>* v, int t, bool act, c2<R>* lds, const c2<R>* __restrict__ tw,
-                                     int twm = 1, int es = 0) {
-  rfft_from<R, N, INV, ES, 0, V, XF>(v, t, act, lds, tw, twm, es);
+                                     int twm = 1, int es = 0, const c2<R>* twr = nullptr) {
+  rfft_from<R, N, INV, ES, 0, V, XF, TWR>(v, t, act, lds, tw, twm, es, twr);
 }
 
 // Element index of register slot s after the transform (last pass outputs) and
@@ -483,6 +488,16 @@ template <int N, int V = 0> constexpr int kOutSlots = LastPass<N, V>::Q * LastPa
 // forward transform's outputs can feed an inverse transform without exchange).
 template <int N, int V = 0> __host__ __device__ constexpr bool rfft_io_same() {
   return plan_np(N, V) == 1 || (plan_np(N, V) == 2 && plan_rad(N, 0, V) == plan_rad(N, 1, V));
+}
+
+// The last pass's twiddles of thread t (one butterfly per thread) into twr[0 .. r-1).
+template <typename R, int N, int V = 0>
+__device__ __forceinline__ void last_pass_twiddles(c2<R>* twr, int t, const c2<R>* __restrict__ tw, int twm) {
+  using C = LastPass<N, V>;
+  static_assert(C::Q == 1 && C::LS > 1, "one butterfly per thread, a pass after the first");
+  const int k = t % C::LS;
+#pragma unroll
+  for (int m = 1; m < C::r; ++m) twr[m - 1] = tw[(m * k * (N / (C::LS * C::r))) * twm];
 }
 
 }  // namespace pf

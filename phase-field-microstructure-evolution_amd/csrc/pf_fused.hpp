@@ -223,6 +223,9 @@ template <typename R> constexpr int kXlineMaxThreads = sizeof(R) == 4 ? 1024 : 5
 // SIMDs with room for a sixth).  FP64: 3 (the 256-point kernel takes 141 VGPRs
 // with no spills).
 template <typename R> constexpr int kXlineWaves = sizeof(R) == 4 ? 5 : 3;
+// 512-point FP32 lines keep their last transform pass's twiddles in registers
+// (kXlineTwiddleRegs): at most 128 VGPRs, which 1024-thread blocks need anyway.
+template <typename R, int NX> constexpr int kXlineWavesN = (sizeof(R) == 4 && NX >= 512) ? 4 : kXlineWaves<R>;
 // Rows each wave keeps in flight when staging a line group.
 constexpr int kXlineRows = 2;
 
@@ -298,12 +301,13 @@ __device__ __forceinline__ void xline_c2r(c2<R>* v, c2<R>* col, const c2<R>* shw
 // thread's outputs at exactly the elements it loaded as that pass's inputs
 // (out_idx = in_idx when j < NB = LS), so no thread overwrites an element
 // another thread has yet to read.
-template <typename R, int M, int V, int XF, bool INV>
-__device__ __forceinline__ void xline_fft(c2<R>* v, c2<R>* col, const c2<R>* shw, int t) {
+template <typename R, int M, int V, int XF, bool INV, bool TWR = false>
+__device__ __forceinline__ void xline_fft(c2<R>* v, c2<R>* col, const c2<R>* shw, int t,
+                                          const c2<R>* twr = nullptr) {
 #pragma unroll
   for (int s = 0; s < kInSlots<M, V>; ++s)
     if (rfft_in_valid<M, V>(t, s)) v[s] = col[lds_swz<XF>(rfft_in_idx<M, V>(t, s))];
-  rfft<R, M, INV, 1, V, XF>(v, t, true, col, shw, 2);
+  rfft<R, M, INV, 1, V, XF, TWR>(v, t, true, col, shw, 2, 0, twr);
 #pragma unroll
   for (int s = 0; s < kOutSlots<M, V>; ++s)
     if (rfft_out_valid<M, V>(t, s)) col[lds_swz<XF>(rfft_out_idx<M, V>(t, s))] = v[s];
@@ -326,6 +330,12 @@ template <int M> __device__ __forceinline__ int xline_partner(int p) { return p 
 // 64-point row's pairs) and pairing in the store alone moved it by less than
 // the run-to-run spread.
 __host__ __device__ constexpr bool xline_pair_rows(int M) { return M >= 256; }
+// 256-point FP32 rows: the last transform pass (radix 16, one butterfly per
+// thread) takes its 15 twiddles, the same for every line and both directions,
+// from registers loaded once per block rather than from the LDS table.  Those
+// blocks hold a CU's LDS alone (37 fields: 85 KB), whose traffic bounds them.
+template <typename R, int M>
+constexpr bool kXlineTwiddleRegs = xline_pair_rows(M) && sizeof(R) == 4 && LastPass<M>::Q == 1 && LastPass<M>::LS > 1;
 template <typename R> __device__ __forceinline__ c2<R> xline_mirror_tw(c2<R> w, int p) {
   return p ? mk<R>(-w.x, w.y) : mk<R>(R(-0.0), R(-1));
 }
@@ -339,7 +349,7 @@ template <typename R> __device__ __forceinline__ const UPar& member_u(const UPar
 }
 
 template <typename R, int NX, int LINES, bool XONLY, bool BATCH = false>
-__global__ void __launch_bounds__(kXlineMaxThreads<R>) __attribute__((amdgpu_waves_per_eu(kXlineWaves<R>)))
+__global__ void __launch_bounds__(kXlineMaxThreads<R>) __attribute__((amdgpu_waves_per_eu(kXlineWavesN<R, NX>)))
 k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __restrict__ mobd,
         const c2<R>* __restrict__ tw, double* __restrict__ part, long ngroups, int diag, int mob,
         const Member<R>* __restrict__ mt) {
@@ -355,7 +365,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   const int njobs = F * LINES;
   c2<R>* sh = reinterpret_cast<c2<R>*>(smem);
   const int rows = xline_rows(M, njobs);
-  constexpr bool PP = xline_pair_rows(M);
+  constexpr bool PP = xline_pair_rows(M), TR = kXlineTwiddleRegs<R, M>;
   c2<R>* shw = sh + (long)rows * MP;  // W_NX^k, k < NX
   R* shn = reinterpret_cast<R*>(shw + NX);  // Nyquist values (without PP)
   double* red = reinterpret_cast<double*>(smem + (((sizeof(c2<R>) * ((size_t)rows * MP + NX) +
@@ -365,6 +375,8 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
   constexpr int KPL = (M + 63) / 64;      // row elements per lane when a wave moves a row
   for (int k = threadIdx.x; k < NX; k += nt) shw[k] = tw[k];
   __syncthreads();  // the staging of the first line group reads the table
+  c2<R> twr[TR ? LastPass<M>::r - 1 : 1];
+  if constexpr (TR) last_pass_twiddles<R, M>(twr, threadIdx.x % T, shw, 2);
   const R scale = (R)u.invP;
   double a_ch = 0.0, a_eta = 0.0;
   R cmax = 0, cmin = 1, pmin = 1, p3 = 0;
@@ -485,7 +497,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
         else xline_c2r<R, M, WV, XF>(v, sh + (njobs - 1) * MP, shw, shn + njobs - 1, lane);
       }
     } else if (!WX || act) {
-      if constexpr (PP) xline_fft<R, M, 0, XF, true>(v, col, shw, t);
+      if constexpr (PP) xline_fft<R, M, 0, XF, true, TR>(v, col, shw, t, twr);
       else xline_c2r<R, M, 0, XF>(v, col, shw, shn + row, t);
     }
     __syncthreads();
@@ -531,7 +543,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
     if (wide) {
       if (TW == 64 || lane < TW) xline_fft<R, M, WV, XF, false>(v, sh + (njobs - 1) * MP, shw, lane);
     } else if (!WX || act) {
-      xline_fft<R, M, 0, XF, false>(v, col, shw, t);
+      xline_fft<R, M, 0, XF, false, TR>(v, col, shw, t, twr);
     }
     __syncthreads();
     // 6. R2C post-processing and store, one row per wave: coalesced along k, or
