@@ -216,8 +216,12 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
 // spare row without wave-local rows).
 // ---------------------------------------------------------------------------
 // Threads per x-line block: FP32 blocks may use 1024 threads, which a 512^3 grid
-// with 37 fields needs (37 transforms x 16 threads); FP64 blocks stop at 512.
+// with 37 fields needs (37 transforms x 16 threads); FP64 blocks stop at 512
+// (640 for 512-point lines, kXlineMaxThreadsN).
 template <typename R> constexpr int kXlineMaxThreads = sizeof(R) == 4 ? 1024 : 512;
+// FP64 blocks of 512-point lines may use 640 threads (37 fields x 16), 3 waves
+// per SIMD at most, which their registers allow.
+template <typename R, int NX> constexpr int kXlineMaxThreadsN = (sizeof(R) == 8 && NX >= 512) ? 640 : kXlineMaxThreads<R>;
 // Minimum waves per SIMD asked of the register allocator.  FP32: 5 (at most 102
 // VGPRs; the 256-point kernel takes 92, so 5 three-wave blocks fit a CU's
 // SIMDs with room for a sixth).  FP64: 3 (the 256-point kernel takes 141 VGPRs
@@ -233,13 +237,21 @@ constexpr int kXlineRows = 2;
 // plan of 96 points spilled 15 VGPRs per lane that way and ran 19% slower at
 // 192^3, so it keeps block barriers, as does 80 (T = 10).
 __host__ __device__ constexpr bool xline_wave_local(int M) { return 64 % plan_T(M) == 0 && plan_np(M) <= 2; }
-__host__ __device__ constexpr int xline_xf(int M) { return xline_wave_local(M) ? (XF_WAVE | XF_SWZ) : 0; }
+// FP64 rows of 256 points are not padded: a 16-byte access serves 16 lanes a
+// cycle, which are one job's T = 16 threads in the exchanges and consecutive
+// elements in the row-wise phases, so the rows need no skew; unpadded, 37 fields
+// fit one block's LDS (148 KiB of rows).
+__host__ __device__ constexpr bool xline_plain_rows(int M, int cbytes) { return cbytes == 16 && M >= 256; }
+__host__ __device__ constexpr int xline_xf(int M, int cbytes) {
+  return xline_wave_local(M) ? (XF_WAVE | (xline_plain_rows(M, cbytes) ? 0 : XF_SWZ)) : 0;
+}
 __host__ __device__ constexpr int xline_rows(int M, int njobs) { return njobs + (xline_wave_local(M) ? 0 : 1); }
 // Row pitch in elements of cbytes bytes: the bank-model choices for FP32
 // elements; FP64 rows take the smallest padded pitch (at 512 points they must
 // fit 33 rows of 16-byte elements in LDS).
 __host__ __device__ constexpr int xline_row_pitch(int M, int cbytes) {
   if (!xline_wave_local(M)) return M | 1;
+  if (xline_plain_rows(M, cbytes)) return M;
   if (cbytes != 8) return M + M / 16 + 1;
   switch (M) {  // smallest of the bank model's best choices, >= M + M / 16
     case 24: return 26;
@@ -349,12 +361,12 @@ template <typename R> __device__ __forceinline__ const UPar& member_u(const UPar
 }
 
 template <typename R, int NX, int LINES, bool XONLY, bool BATCH = false>
-__global__ void __launch_bounds__(kXlineMaxThreads<R>) __attribute__((amdgpu_waves_per_eu(kXlineWavesN<R, NX>)))
+__global__ void __launch_bounds__((kXlineMaxThreadsN<R, NX>)) __attribute__((amdgpu_waves_per_eu(kXlineWavesN<R, NX>)))
 k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __restrict__ mobd,
         const c2<R>* __restrict__ tw, double* __restrict__ part, long ngroups, int diag, int mob,
         const Member<R>* __restrict__ mt) {
   constexpr int M = NX / 2, T = plan_T(M);
-  constexpr int MP = xline_row_pitch(M, sizeof(c2<R>)), XF = xline_xf(M);
+  constexpr int MP = xline_row_pitch(M, sizeof(c2<R>)), XF = xline_xf(M, sizeof(c2<R>));
   constexpr bool WX = xline_wave_local(M), HAVEW = xline_has_wide(M);
   // the wide plan's variant (3) and threads; the T-thread plan stands in when
   // this length has no wide plan, and that branch is never taken then

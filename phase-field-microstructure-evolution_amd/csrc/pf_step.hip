@@ -847,7 +847,7 @@ bool xline_fits(const Geo& g, long* nt_out, size_t* shb_out) {
   const size_t body = sizeof(c2<R>) * ((size_t)xline_rows(M, (int)jobs) * MP + NX) +
                       (xline_pair_rows(M) ? 0 : sizeof(R) * (size_t)(jobs + 1));
   const size_t shb = ((body + 15) & ~(size_t)15) + 16 * sizeof(double);
-  if (need > kXlineMaxThreads<R> || shb > kXlineMaxLds) return false;
+  if (need > kXlineMaxThreadsN<R, NX> || shb > kXlineMaxLds) return false;
   *nt_out = need;
   *shb_out = shb;
   return true;

@@ -136,18 +136,23 @@ def test_fp32_long_x_lines_many_fields(pf, oracle):
 
 def test_fp32_long_x_lines_several_groups_per_block(pf, oracle):
     """As above with 1024 lines over 256 blocks: each block takes four line groups
-    and loads the next group's rows while it works on the current one."""
+    in turn (with D_gb != 0, the mobility chain's one-field blocks too)."""
     _compare(pf, oracle, (512, 32, 32), 36, 40, 4, "fp32_mixed", dict(VOR, D_gb=0.25), 2e-6, 2e-6,
              tuning=dict(xline_grid=256))
 
 
-def test_fp64_reduced_too_many_fields_is_an_error(pf):
-    """FP64 x-line blocks stop at 512 threads: 37 fields of 512-point lines need 592."""
-    phi, con = pf.ic_voronoi(512, 16, 16, 36, 40)
+def test_fp64_long_x_lines_many_fields(pf, oracle):
+    """FP64 x lines of 512 points with 37 fields: 640-thread blocks and unpadded
+    rows (156 KiB of LDS), against the oracle."""
+    _compare(pf, oracle, (512, 16, 16), 36, 40, 4, "fp64_reduced", dict(VOR, D_gb=0.25), 1e-13, 1e-13)
+
+
+def test_fp64_too_many_fields_is_an_error(pf):
+    """41 fields of 512-point FP64 lines need 656 threads, past the 640 an FP64
+    block holds: refused when the handle is created."""
     with pytest.raises(pf.PfError) as e:
-        with pf.Simulation(pf.make_params(**VOR), 512, 16, 16, 36, "fp64_reduced") as sim:
-            sim.put_state(phi, con)
-            sim.step(1)
+        with pf.Simulation(pf.make_params(**VOR), 512, 8, 8, 40, "fp64_reduced"):
+            pass
     assert e.value.code == pf.PF_ENOTIMPL
 
 
