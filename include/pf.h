@@ -66,7 +66,9 @@ typedef struct pf_handle pf_handle;
 
 /* ---- timestep ------------------------------------------------------------
  * Replaces the body of the time loop (src/var_diff.f90:556-1118: loops 1-14,
- * energies, extrema) and the P3DFFT setup/teardown (:128-138, :1162). */
+ * energies, extrema) and the P3DFFT setup/teardown (:128-138, :1162).
+ * pf_create returns PF_ENOTIMPL when one x-line block cannot hold every field of
+ * a line (512-point lines: up to 64 fields in FP32-mixed, 37 in FP64 reduced). */
 int pf_create(const pf_params* p, int nx, int ny, int nz, int var, int mode, pf_handle** out);
 int pf_destroy(pf_handle* h);
 /* Upload / download the state (src/var_diff.f90:154,143 layouts). */
