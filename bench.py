@@ -283,34 +283,38 @@ def boundary(pf, torch, n, var, phi, con):
     print(json.dumps({"boundary": res}), flush=True)
 
 
-def ensemble(pf, torch, args):
-    """Parameter-sweep ensemble: aggregate grid-point-updates/s of N simulations
-    advanced together (pf_ensemble_step), after 20 untimed steps."""
+def ensemble_measure(pf, torch, n_members, steps, graphs=True):
+    """Parameter-sweep ensemble: aggregate grid-point-updates/s of n_members
+    simulations advanced together (pf_ensemble_step), after 20 untimed steps."""
     import pf_amd.runconfig as rc
 
-    members = rc.ensemble_members(pf, n=args.ensemble, var=8)
+    members = rc.ensemble_members(pf, n=n_members, var=8)
     pts = sum(m["grid"] ** 3 for m in members)
     log(f"ensemble: {len(members)} simulations, {pts / 1e6:.1f} M points in total")
     sims = rc.run_ensemble(pf, members, 20)
     for s in sims:
-        s.set_graphs(not args.no_graphs)
+        s.set_graphs(graphs)
     try:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        pf.ensemble_step(sims, args.ensemble_steps)
+        pf.ensemble_step(sims, steps)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
     finally:
         for s in sims:
             s.close()
-    gpups = pts * args.ensemble_steps / dt
-    log(f"ensemble: {dt / args.ensemble_steps * 1e3:.3f} ms per ensemble step, {gpups:.3e} GPUPS")
-    print(json.dumps({"metric": "grid-point-updates/s, parameter-sweep ensemble", "value": gpups,
-                      "unit": "grid-point-updates/s", "members": len(members), "points": pts,
-                      "steps": args.ensemble_steps, "ms_per_step": dt / args.ensemble_steps * 1e3,
-                      "dtype": "f32", "config": {"workload": f"{len(members)} Voronoi polycrystals "
-                      "(16 seeds each), grids 64/96/128/160/192^3 in turn, c + 8 eta, dt=0.005, kappa~U(0.25,2), "
-                      "M~U(0.5,2), L~U(0.5,2) from one ran_2 stream"}}), flush=True)
+    gpups = pts * steps / dt
+    log(f"ensemble: {dt / steps * 1e3:.3f} ms per ensemble step, {gpups:.3e} GPUPS")
+    return {"value": gpups, "unit": "grid-point-updates/s", "members": len(members), "points": pts,
+            "steps": steps, "ms_per_step": dt / steps * 1e3, "dtype": "f32",
+            "config": {"workload": f"{len(members)} Voronoi polycrystals (16 seeds each), grids "
+                       "64/96/128/160/192^3 in turn, c + 8 eta, dt=0.005, kappa~U(0.25,2), M~U(0.5,2), "
+                       "L~U(0.5,2) from one ran_2 stream"}}
+
+
+def ensemble(pf, torch, args):
+    r = ensemble_measure(pf, torch, args.ensemble, args.ensemble_steps, not args.no_graphs)
+    print(json.dumps(dict({"metric": "grid-point-updates/s, parameter-sweep ensemble"}, **r)), flush=True)
 
 
 def main():
@@ -338,6 +342,10 @@ def main():
                     help="instead of the single grid: N independent FP32 simulations, grids cycling "
                          "64..192^3, var 8, parameters from one ran_2 stream, advanced together")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 48-member ensemble leg of the default run (its figure goes into the "
+                         "line's 'ensemble' key)")
+    ap.add_argument("--ensemble-leg-steps", type=int, default=100)
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel instead of replaying captured step periods")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -421,6 +429,10 @@ def main():
         log(f"cpu baseline: {cpu['value']:.3e} GPUPS on {cpu['cores']} threads "
             f"({cpu['host']['cpu_model']}), 1 thread {cpu['one_thread']['value']:.3e}")
 
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble_measure(pf, torch, 48, args.ensemble_leg_steps, not args.no_graphs)
+
     # whole-step HBM model (reduced, constant mobility): F(4 B_r + 13 B_c) per step
     F = var + 1
     Br = n ** 3 * 4.0
@@ -468,6 +480,7 @@ def main():
         "launch_shapes": tuning,
         "fp64": fp64,
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     if "fp64_faithful" in fp64:
         out["ratio_vs_fp64_faithful"] = gpups / fp64["fp64_faithful"]["value"]
