@@ -352,6 +352,15 @@ template <typename R> __device__ __forceinline__ c2<R> xline_mirror_tw(c2<R> w, 
   return p ? mk<R>(-w.x, w.y) : mk<R>(R(-0.0), R(-1));
 }
 
+// Lane t of a 16-lane row receives x from lane (16 - t) mod 16 of the same row:
+// DPP row_mirror (lane 15 - t) then row_ror:1 (from lane t - 1), register moves
+// with no LDS traffic.
+__device__ __forceinline__ float dpp_partner16(float x) {
+  const int m = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x140, 0xF, 0xF, false);
+  const int r = __builtin_amdgcn_update_dpp(0, m, 0x121, 0xF, 0xF, false);
+  return __builtin_bit_cast(float, r);
+}
+
 // BATCH: ngroups counts the line groups of every member in mt (member m owns
 // groups m*gpm .. m*gpm + gpm - 1, gpm = plane / LINES), each taking its banks
 // and parameters from its entry; wm, wn, u are unused; no diagnostics.
@@ -551,17 +560,46 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       }
     }
     __syncthreads();
-    // 5. forward x transform of each job
+    // 5. forward x transform of each job.  With TR (256-point FP32 rows, 16
+    //    threads a job, one 16-lane DPP row each) a job's threads also do its R2C
+    //    post-processing and store: after the last pass thread t holds
+    //    Z[t + 16 s] in slot s, whose partner Z[M - t - 16 s] is slot 15 - s of
+    //    thread 16 - t (for t = 0: its own slot (16 - s) mod 16), so the pairs
+    //    come by DPP instead of a write-back and a second read of the row (512^3
+    //    x 37: x line 13.5-13.8 -> 13.0-13.1 ms).  The lone wide job, if any,
+    //    still goes through its row and phase 6.
+    const bool wide_any = HAVEW && njobs % JW == 1;
     if (wide) {
       if (TW == 64 || lane < TW) xline_fft<R, M, WV, XF, false>(v, sh + (njobs - 1) * MP, shw, lane);
     } else if (!WX || act) {
-      xline_fft<R, M, 0, XF, false, TR>(v, col, shw, t, twr);
+      if constexpr (TR) {
+        static_assert(T == 16 && kInSlots<M> == 16 && kOutSlots<M> == 16, "16 threads, 16 slots a job");
+#pragma unroll
+        for (int s = 0; s < 16; ++s) v[s] = col[lds_swz<XF>(rfft_in_idx<M>(t, s))];
+        rfft<R, M, false, 1, 0, XF, true>(v, t, true, col, shw, 2, 0, twr);
+        const int f = job / LINES, l = job - f * LINES;
+        c2<R>* dst = wmm + (long)f * g.hmain + (L0 + l) * M;
+        const R hs = R(0.5) * scale;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+          const int k = rfft_out_idx<M>(t, s);  // t + 16 s
+          const c2<R> src = v[15 - s];
+          c2<R> pr = mk<R>(dpp_partner16(src.x), dpp_partner16(src.y));
+          if (t == 0) pr = v[(16 - s) % 16];
+          st_nt<R>(dst + k, r2c_post<R>(v[s], pr, shw[k], hs));
+        }
+        if (t == 0) wnn[(long)f * g.plane + L0 + l] = mk<R>((v[0].x - v[0].y) * scale, R(0));
+      } else {
+        xline_fft<R, M, 0, XF, false, TR>(v, col, shw, t, twr);
+      }
     }
     __syncthreads();
     // 6. R2C post-processing and store, one row per wave: coalesced along k, or
-    //    with the pairs (p, M - p) of 1 per lane, along k and M - k
+    //    with the pairs (p, M - p) of 1 per lane, along k and M - k (with TR only
+    //    the wide job's row)
     const R hs = R(0.5) * scale;
     for (int jb = wv; jb < njobs; jb += nw) {
+      if (TR && !(wide_any && jb == njobs - 1)) continue;
       const int f = jb / LINES, l = jb - f * LINES;
       c2<R>* dst = wmm + (long)f * g.hmain + (L0 + l) * M;
       const c2<R>* rw = sh + jb * MP;
@@ -586,6 +624,7 @@ k_xline(Geo g, UPar u, c2<R>* __restrict__ wm, c2<R>* __restrict__ wn, R* __rest
       }
     }
     for (int i = tid; i < njobs; i += nt) {
+      if (TR && !(wide_any && i == njobs - 1)) continue;
       const int f = i / LINES, l = i - f * LINES;
       const c2<R> z0 = sh[i * MP];
       wnn[(long)f * g.plane + L0 + l] = mk<R>((z0.x - z0.y) * scale, R(0));
