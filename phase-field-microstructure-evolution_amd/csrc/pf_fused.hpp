@@ -208,7 +208,9 @@ __device__ __forceinline__ void nl_points(typename Pts<R, NPT>::T* v, long fs, i
 // 256-point rows they also do the real <-> half-complex pairing, a lane taking
 // elements k and M - k together (xline_partner, xline_pair_rows); shorter rows
 // are paired by the transform threads as they load (C2R) and by the store's
-// lanes (R2C), with the Nyquist values staged in LDS.
+// lanes (R2C), with the Nyquist values staged in LDS.  256-point FP32 rows
+// (kXlineTwiddleRegs) skip the final store's row pass: each job's 16 threads
+// pair their forward transform's outputs by DPP and store them (phase 5).
 // The table W_NX^k, k < NX, is copied to LDS once per block: the pairing reads
 // k < M, the x transforms (length M, stride 2) the rest.
 // Dynamic LDS: (rows*MP + NX) complex (+ njobs + 1 reals for the Nyquist values
